@@ -1,0 +1,770 @@
+// Grid, batched y/z pencil FFT passes, x-line transforms and the drop-in
+// spectral operators (fft_t, spec_t, stress_t of the reference) on FP64
+// device buffers in the reference layouts.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "hy_internal.h"
+
+namespace hy {
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_last_error;
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+// ------------------------------------------------------------------- grid
+const TwiddleSet& Grid::tw(int L) const {
+  for (const auto& t : tws)
+    if (t.L == L) return t;
+  return tws.front();
+}
+
+static int wrapi(int m, int n) { return m <= n / 2 ? m : m - n; }  // grid.f90:116-123
+
+int grid_build(Grid& g, int nx, int ny, int nz, double lx, double ly, double lz, int dealias,
+               int precision) {
+  if (nx < 2) return set_error(HY_ERR_INVALID, "grid: nx must be >= 2");
+  if (nx % 2) return set_error(HY_ERR_INVALID, "grid: nx must be even");
+  if (ny > 1 && ny % 2) return set_error(HY_ERR_INVALID, "grid: ny must be even (or 1)");
+  if (nz > 1 && nz % 2) return set_error(HY_ERR_INVALID, "grid: nz must be even (or 1)");
+  if (ny < 1 || nz < 1) return set_error(HY_ERR_INVALID, "grid: ny, nz must be >= 1");
+  if (!supported_len(nx) || (ny > 1 && !supported_len(ny)) || (nz > 1 && !supported_len(nz)))
+    return set_error(HY_ERR_UNSUPPORTED,
+                     "grid: each of nx, ny, nz must be 1 (ny, nz) or one of 4 8 12 16 24 32 48 64 96 128 256");
+  if (precision != HY_PREC_FP64 && precision != HY_PREC_MIXED)
+    return set_error(HY_ERR_INVALID, "grid: unknown precision");
+  const double twopi = 2.0 * 3.14159265358979323846264338327950288;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.nkx = nx / 2 + 1;
+  g.ndim = 1;
+  if (ny > 1) g.ndim = 2;
+  if (nz > 1) g.ndim = 3;
+  g.lx = lx; g.ly = ly; g.lz = lz;
+  g.dx = lx / nx; g.dy = ly / ny; g.dz = lz / nz;
+  g.volume = lx;
+  if (ny > 1) g.volume *= ly;
+  if (nz > 1) g.volume *= lz;
+  g.dealias = dealias ? 1 : 0;
+  g.precision = precision;
+  g.kx.assign(g.nkx, 0.0); g.ky.assign(ny, 0.0); g.kz.assign(nz, 0.0);
+  for (int i = 0; i < g.nkx; ++i) g.kx[i] = twopi * i / lx;
+  if (ny > 1) for (int j = 0; j < ny; ++j) g.ky[j] = twopi * wrapi(j, ny) / ly;
+  if (nz > 1) for (int k = 0; k < nz; ++k) g.kz[k] = twopi * wrapi(k, nz) / lz;
+  // The reference mask (grid.f90:90-108) is a per-axis test, so the kept set is
+  // a box: kx index < Kx, |my| <= My, |mz| <= Mz.  Derive the box with the same
+  // floating-point comparisons and verify it point by point.
+  const double huge = 1.7976931348623157e308;
+  double kxc = twopi * (nx / 2) / lx * (2.0 / 3.0), kyc = huge, kzc = huge;
+  if (ny > 1) kyc = twopi * (ny / 2) / ly * (2.0 / 3.0);
+  if (nz > 1) kzc = twopi * (nz / 2) / lz * (2.0 / 3.0);
+  auto keepx = [&](int i) { return dealias ? !(std::fabs(g.kx[i]) >= kxc) : (i != g.nkx - 1); };
+  auto keepy = [&](int j) { return dealias ? !(std::fabs(g.ky[j]) >= kyc) : !(ny > 1 && j == ny / 2); };
+  auto keepz = [&](int k) { return dealias ? !(std::fabs(g.kz[k]) >= kzc) : !(nz > 1 && k == nz / 2); };
+  g.Kx = 0;
+  while (g.Kx < g.nkx && keepx(g.Kx)) ++g.Kx;
+  for (int i = g.Kx; i < g.nkx; ++i)
+    if (keepx(i)) return set_error(HY_ERR_INVALID, "grid: kept kx set is not contiguous");
+  g.My = 0;
+  if (ny > 1) { while (g.My + 1 <= ny / 2 && keepy(g.My + 1)) ++g.My; }
+  g.Mz = 0;
+  if (nz > 1) { while (g.Mz + 1 <= nz / 2 && keepz(g.Mz + 1)) ++g.Mz; }
+  for (int j = 0; j < ny; ++j) {
+    bool box = std::abs(wrapi(j, ny)) <= g.My;
+    if (keepy(j) != box) return set_error(HY_ERR_INVALID, "grid: kept ky set is not a box");
+  }
+  for (int k = 0; k < nz; ++k) {
+    bool box = std::abs(wrapi(k, nz)) <= g.Mz;
+    if (keepz(k) != box) return set_error(HY_ERR_INVALID, "grid: kept kz set is not a box");
+  }
+  if (g.Kx < 1 || !keepy(0) || !keepz(0)) return set_error(HY_ERR_INVALID, "grid: empty kept set");
+  g.Ky = ny > 1 ? 2 * g.My + 1 : 1;
+  g.Kz = nz > 1 ? 2 * g.Mz + 1 : 1;
+  g.kmax_dealiased = 0.0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < g.nkx; ++i)
+        if (keepx(i) && keepy(j) && keepz(k)) {
+          double k2 = g.kx[i] * g.kx[i] + g.ky[j] * g.ky[j] + g.kz[k] * g.kz[k];
+          g.kmax_dealiased = std::max(g.kmax_dealiased, std::sqrt(k2));
+        }
+  // device wavenumbers
+  HY_HIP(hipMalloc(&g.d_kx, sizeof(double) * g.nkx));
+  HY_HIP(hipMalloc(&g.d_ky, sizeof(double) * ny));
+  HY_HIP(hipMalloc(&g.d_kz, sizeof(double) * nz));
+  HY_HIP(hipMemcpy(g.d_kx, g.kx.data(), sizeof(double) * g.nkx, hipMemcpyHostToDevice));
+  HY_HIP(hipMemcpy(g.d_ky, g.ky.data(), sizeof(double) * ny, hipMemcpyHostToDevice));
+  HY_HIP(hipMemcpy(g.d_kz, g.kz.data(), sizeof(double) * nz, hipMemcpyHostToDevice));
+  // twiddles exp(-2 pi i j/L), computed in extended precision and rounded once
+  int lens[3] = {nx, ny, nz};
+  for (int L : lens) {
+    if (L < 2) continue;
+    bool have = false;
+    for (auto& t : g.tws) have |= (t.L == L);
+    if (have) continue;
+    TwiddleSet t;
+    t.L = L;
+    std::vector<c64> h64(L);
+    std::vector<c32> h32(L);
+    const long double tp = 6.283185307179586476925286766559005768L;
+    for (int j = 0; j < L; ++j) {
+      long double a = tp * (long double)j / (long double)L;
+      double c = (double)cosl(a), s = (double)-sinl(a);
+      h64[j] = mk<double>(c, s);
+      h32[j] = mk<float>((float)cosl(a), (float)-sinl(a));
+    }
+    HY_HIP(hipMalloc(&t.d64, sizeof(c64) * L));
+    HY_HIP(hipMalloc(&t.d32, sizeof(c32) * L));
+    HY_HIP(hipMemcpy(t.d64, h64.data(), sizeof(c64) * L, hipMemcpyHostToDevice));
+    HY_HIP(hipMemcpy(t.d32, h32.data(), sizeof(c32) * L, hipMemcpyHostToDevice));
+    g.tws.push_back(t);
+  }
+  return HY_OK;
+}
+
+void grid_free(Grid& g) {
+  (void)hipFree(g.d_kx); (void)hipFree(g.d_ky); (void)hipFree(g.d_kz);
+  for (auto& t : g.tws) { (void)hipFree(t.d64); (void)hipFree(t.d32); }
+  g.tws.clear();
+  if (g.d_scratch) (void)hipFree(g.d_scratch);
+  g.d_scratch = nullptr;
+  g.d_kx = g.d_ky = g.d_kz = nullptr;
+}
+
+// ---------------------------------------------------- batched pencil passes
+struct PencilArgs {
+  PencilField f[MAXPF];
+  int A, B, L, K, h;        // contiguous count, outer count, FFT length, stored count, #nonneg stored
+  double scale;             // forward: 1/L
+  const double* kax;        // wavenumbers along the pass axis in FFT order (L entries)
+  // decoding of the contiguous index for MUL_NEGK2 (first inverse pass only):
+  const double* kxv; int ax_x;   // kx array; x count in the contiguous index
+  const double* kyv; int ay_n;   // ky array (FFT order, ny entries); y stored count in the contiguous index (0 = none)
+  int ay_h, ay_L;                // y stored-index mapping (h, L) when ay_n > 0
+  int y_from_pencil;             // 1: ky comes from the pencil position (y pass)
+};
+
+template <class T>
+__device__ inline cpx<T> apply_mul(cpx<T> v, int mul, int p, int ga, const PencilArgs& a) {
+  if (mul == MUL_IK) {
+    double k = a.kax[p];
+    return mk<T>((T)(-k * (double)v.y), (T)(k * (double)v.x));
+  }
+  if (mul == MUL_NEGK2) {
+    int ix = ga % a.ax_x;
+    double kx = a.kxv[ix];
+    double ky = 0.0, kz = 0.0;
+    if (a.y_from_pencil) {
+      ky = a.kax[p];
+    } else {
+      kz = a.kax[p];
+      if (a.ay_n > 0) {
+        int iy = ga / a.ax_x;
+        int jy = iy < a.ay_h ? iy : iy + (a.ay_L - a.ay_n);
+        ky = a.kyv[jy];
+      }
+    }
+    double k2 = kx * kx + ky * ky + kz * kz;
+    return mk<T>((T)(-k2 * (double)v.x), (T)(-k2 * (double)v.y));
+  }
+  return v;
+}
+
+template <class T>
+__device__ inline cpx<T> load_any(const void* p, size_t i, int is64) {
+  if (is64) return cvt<T>(reinterpret_cast<const c64*>(p)[i]);
+  return cvt<T>(reinterpret_cast<const c32*>(p)[i]);
+}
+
+template <class T, int L, int NA, int NT>
+__global__ __launch_bounds__(NT) void k_pencil_inv(PencilArgs a, const cpx<T>* __restrict__ twg) {
+  __shared__ cpx<T> buf[L * NA];
+  __shared__ cpx<T> tw[L];
+  const int tid = threadIdx.x;
+  const int a0 = blockIdx.x * NA, b = blockIdx.y;
+  const PencilField& f = a.f[blockIdx.z];
+  for (int j = tid; j < L; j += NT) tw[j] = twg[j];
+  const int tailoff = L - a.K;
+  for (int e = tid; e < L * NA; e += NT) {
+    const int p = e / NA, al = e % NA, ga = a0 + al;
+    int t = -1;
+    if (p < a.h) t = p;
+    else if (p >= a.h + tailoff) t = p - tailoff;
+    cpx<T> v = mk<T>(0, 0);
+    if (t >= 0 && ga < a.A) {
+      v = load_any<T>(f.in, ((size_t)b * a.K + t) * a.A + ga, f.in_f64);
+      if (f.k0_offset != 0.0 && t == 0 && ga == 0 && b == 0) v.x = (T)((double)v.x + f.k0_offset);
+      v = apply_mul<T>(v, f.mul, p, ga, a);
+    }
+    buf[e] = v;
+  }
+  __syncthreads();
+  fft_lds<T, L, NA, NT, true>(buf, tw, tid);
+  cpx<T>* out = reinterpret_cast<cpx<T>*>(f.out);
+  for (int e = tid; e < L * NA; e += NT) {
+    const int p = e / NA, al = e % NA, ga = a0 + al;
+    if (ga < a.A) out[((size_t)b * L + p) * a.A + ga] = buf[e];
+  }
+}
+
+template <class T, int L, int NA, int NT>
+__global__ __launch_bounds__(NT) void k_pencil_fwd(PencilArgs a, const cpx<T>* __restrict__ twg) {
+  __shared__ cpx<T> buf[L * NA];
+  __shared__ cpx<T> tw[L];
+  const int tid = threadIdx.x;
+  const int a0 = blockIdx.x * NA, b = blockIdx.y;
+  const PencilField& f = a.f[blockIdx.z];
+  for (int j = tid; j < L; j += NT) tw[j] = twg[j];
+  for (int e = tid; e < L * NA; e += NT) {
+    const int p = e / NA, al = e % NA, ga = a0 + al;
+    buf[e] = ga < a.A ? load_any<T>(f.in, ((size_t)b * L + p) * a.A + ga, f.in_f64) : mk<T>(0, 0);
+  }
+  __syncthreads();
+  fft_lds<T, L, NA, NT, false>(buf, tw, tid);
+  cpx<T>* out = reinterpret_cast<cpx<T>*>(f.out);
+  const int tailoff = L - a.K;
+  const T sc = (T)a.scale;
+  for (int e = tid; e < a.K * NA; e += NT) {
+    const int t = e / NA, al = e % NA, ga = a0 + al;
+    if (ga >= a.A) continue;
+    const int p = t < a.h ? t : t + tailoff;
+    cpx<T> v = scal(buf[p * NA + al], sc);
+    v = apply_mul<T>(v, f.mul, p, ga, a);
+    out[((size_t)b * a.K + t) * a.A + ga] = v;
+  }
+}
+
+template <class T>
+constexpr int pencil_na(int L) {
+  int na = (sizeof(T) == 16 ? 2048 : 4096) / L;
+  return na < 4 ? 4 : (na > 64 ? 64 : na);
+}
+template <class T>
+constexpr int pencil_nt(int L) {
+  int nt = L * pencil_na<T>(L) / 4;
+  return nt < 64 ? 64 : (nt > 256 ? 256 : nt);
+}
+
+template <class T, int L>
+static void launch_pencil(const PencilArgs& a, bool inverse, int nf, const cpx<T>* tw, hipStream_t s) {
+  constexpr int NA = pencil_na<cpx<T>>(L);
+  constexpr int NT = pencil_nt<cpx<T>>(L);
+  dim3 grid((a.A + NA - 1) / NA, a.B, nf);
+  if (inverse) hipLaunchKernelGGL((k_pencil_inv<T, L, NA, NT>), grid, dim3(NT), 0, s, a, tw);
+  else hipLaunchKernelGGL((k_pencil_fwd<T, L, NA, NT>), grid, dim3(NT), 0, s, a, tw);
+}
+
+template <class T>
+static int dispatch_pencil(const PencilArgs& a, bool inverse, int nf, const cpx<T>* tw, hipStream_t s) {
+  switch (a.L) {
+#define HY_PL(LL) case LL: launch_pencil<T, LL>(a, inverse, nf, tw, s); break;
+    HY_PL(4) HY_PL(8) HY_PL(12) HY_PL(16) HY_PL(24) HY_PL(32) HY_PL(48) HY_PL(64) HY_PL(96)
+    HY_PL(128) HY_PL(256)
+#undef HY_PL
+    default: return set_error(HY_ERR_UNSUPPORTED, "pencil: unsupported length");
+  }
+  HY_HIP(hipGetLastError());
+  return HY_OK;
+}
+
+int pencil_pass(const Grid& g, int axis, bool inverse, bool trunc, int A, int B,
+                const PencilField* f, int nf, hipStream_t s) {
+  if (nf <= 0) return HY_OK;
+  PencilArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.A = A;
+  a.B = B;
+  a.L = axis == 1 ? g.ny : g.nz;
+  a.K = trunc ? (axis == 1 ? g.Ky : g.Kz) : a.L;
+  a.h = trunc ? (a.K + 1) / 2 : a.L;
+  a.scale = 1.0 / a.L;
+  a.kax = axis == 1 ? g.d_ky : g.d_kz;
+  a.kxv = g.d_kx;
+  a.ax_x = trunc ? g.Kx : g.nkx;
+  a.kyv = g.d_ky;
+  a.y_from_pencil = axis == 1 ? 1 : 0;
+  if (axis == 2 && g.ny > 1) {
+    a.ay_n = trunc ? g.Ky : g.ny;
+    a.ay_h = trunc ? (g.Ky + 1) / 2 : g.ny;
+    a.ay_L = g.ny;
+  }
+  const bool f32 = (g.precision == HY_PREC_MIXED) && trunc;
+  for (int i0 = 0; i0 < nf; i0 += MAXPF) {
+    int n = std::min(MAXPF, nf - i0);
+    for (int i = 0; i < n; ++i) a.f[i] = f[i0 + i];
+    const TwiddleSet& t = g.tw(a.L);
+    if (f32) HY_TRY(dispatch_pencil<float>(a, inverse, n, t.d32, s));
+    else HY_TRY(dispatch_pencil<double>(a, inverse, n, t.d64, s));
+  }
+  return HY_OK;
+}
+
+// ------------------------------------------------------ plain x-line passes
+// Two real lines per block packed as one complex FFT of length NX.
+template <int NX>
+__global__ __launch_bounds__(64) void k_xplain_r2c(const double* __restrict__ u, c64* __restrict__ out,
+                                                   int nlines, int Kout, double scale,
+                                                   const c64* __restrict__ twg) {
+  __shared__ c64 buf[NX];
+  __shared__ c64 tw[NX];
+  const int tid = threadIdx.x;
+  const int l0 = blockIdx.x * 2, l1 = l0 + 1;
+  const bool has1 = l1 < nlines;
+  for (int j = tid; j < NX; j += 64) tw[j] = twg[j];
+  for (int x = tid; x < NX; x += 64)
+    buf[x] = mk<double>(u[(size_t)l0 * NX + x], has1 ? u[(size_t)l1 * NX + x] : 0.0);
+  __syncthreads();
+  fft_lds<double, NX, 1, 64, false>(buf, tw, tid);
+  for (int k = tid; k < Kout; k += 64) {
+    c64 Z = buf[k], Zc = conjc(buf[(NX - k) % NX]);
+    c64 A = scal(Z + Zc, 0.5 * scale);
+    c64 D = Z - Zc;
+    out[(size_t)l0 * Kout + k] = A;
+    if (has1) out[(size_t)l1 * Kout + k] = mk<double>(0.5 * scale * D.y, -0.5 * scale * D.x);
+  }
+}
+
+template <int NX>
+__global__ __launch_bounds__(64) void k_xplain_c2r(const c64* __restrict__ in, double* __restrict__ u,
+                                                   int nlines, int Kin, const c64* __restrict__ twg) {
+  __shared__ c64 buf[NX];
+  __shared__ c64 tw[NX];
+  const int tid = threadIdx.x;
+  const int l0 = blockIdx.x * 2, l1 = l0 + 1;
+  const bool has1 = l1 < nlines;
+  for (int j = tid; j < NX; j += 64) tw[j] = twg[j];
+  for (int k = tid; k < NX; k += 64) {
+    const int kk = k <= NX / 2 ? k : NX - k;
+    c64 A = kk < Kin ? in[(size_t)l0 * Kin + kk] : mk<double>(0, 0);
+    c64 B = (has1 && kk < Kin) ? in[(size_t)l1 * Kin + kk] : mk<double>(0, 0);
+    if (kk == 0 || 2 * kk == NX) { A.y = 0; B.y = 0; }
+    if (k > NX / 2) { A = conjc(A); B = conjc(B); }
+    buf[k] = mk<double>(A.x - B.y, A.y + B.x);
+  }
+  __syncthreads();
+  fft_lds<double, NX, 1, 64, true>(buf, tw, tid);
+  for (int x = tid; x < NX; x += 64) {
+    u[(size_t)l0 * NX + x] = buf[x].x;
+    if (has1) u[(size_t)l1 * NX + x] = buf[x].y;
+  }
+}
+
+static int xplain(const Grid& g, bool fwd, const void* in, void* out, int K, hipStream_t s) {
+  const int nlines = g.ny * g.nz;
+  dim3 grid((nlines + 1) / 2);
+  const c64* tw = g.tw(g.nx).d64;
+  switch (g.nx) {
+#define HY_XP(LL)                                                                             \
+  case LL:                                                                                    \
+    if (fwd) hipLaunchKernelGGL((k_xplain_r2c<LL>), grid, dim3(64), 0, s, (const double*)in,  \
+                                (c64*)out, nlines, K, 1.0 / LL, tw);                          \
+    else hipLaunchKernelGGL((k_xplain_c2r<LL>), grid, dim3(64), 0, s, (const c64*)in,         \
+                            (double*)out, nlines, K, tw);                                     \
+    break;
+    HY_XP(4) HY_XP(8) HY_XP(12) HY_XP(16) HY_XP(24) HY_XP(32) HY_XP(48) HY_XP(64) HY_XP(96)
+    HY_XP(128) HY_XP(256)
+#undef HY_XP
+    default: return set_error(HY_ERR_UNSUPPORTED, "x pass: unsupported length");
+  }
+  HY_HIP(hipGetLastError());
+  return HY_OK;
+}
+
+// Full-layout (nkx, ny, nz) transforms used by the drop-in operators.
+// r2c: u (real) -> uhat;  c2r: work (spectrum, overwritten) -> u.
+static int fft_r2c_full(const Grid& g, const double* u, c64* uhat, hipStream_t s) {
+  HY_TRY(xplain(g, true, u, uhat, g.nkx, s));
+  if (g.ny > 1) {
+    PencilField f{uhat, uhat, MUL_NONE, 1, 0.0};
+    HY_TRY(pencil_pass(g, 1, false, false, g.nkx, g.nz, &f, 1, s));
+  }
+  if (g.nz > 1) {
+    PencilField f{uhat, uhat, MUL_NONE, 1, 0.0};
+    HY_TRY(pencil_pass(g, 2, false, false, g.nkx * g.ny, 1, &f, 1, s));
+  }
+  return HY_OK;
+}
+static int fft_c2r_full_inplace(const Grid& g, c64* work, double* u, hipStream_t s) {
+  if (g.nz > 1) {
+    PencilField f{work, work, MUL_NONE, 1, 0.0};
+    HY_TRY(pencil_pass(g, 2, true, false, g.nkx * g.ny, 1, &f, 1, s));
+  }
+  if (g.ny > 1) {
+    PencilField f{work, work, MUL_NONE, 1, 0.0};
+    HY_TRY(pencil_pass(g, 1, true, false, g.nkx, g.nz, &f, 1, s));
+  }
+  return xplain(g, false, work, u, g.nkx, s);
+}
+
+// --------------------------------------------- full-spectrum multiply kernels
+enum SpecOp { SOP_COPY = 0, SOP_MASK, SOP_IK_MASK, SOP_NEGK2_MASK, SOP_KERNEL_MASK, SOP_ACC_IK };
+
+struct FullSpecArgs {
+  int nkx, ny, nz, Kx, My, Mz;
+  const double *kx, *ky, *kz;
+};
+
+__device__ inline bool full_kept(const FullSpecArgs& a, int i, int j, int k) {
+  int my = j <= a.ny / 2 ? j : j - a.ny;
+  int mz = k <= a.nz / 2 ? k : k - a.nz;
+  return i < a.Kx && (my < 0 ? -my : my) <= a.My && (mz < 0 ? -mz : mz) <= a.Mz;
+}
+
+__global__ void k_specop(FullSpecArgs a, int op, int dir, const c64* __restrict__ in,
+                         const double* __restrict__ kern, c64* __restrict__ out) {
+  size_t n = (size_t)a.nkx * a.ny * a.nz;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    int i = e % a.nkx;
+    int j = (e / a.nkx) % a.ny;
+    int k = e / ((size_t)a.nkx * a.ny);
+    double m = full_kept(a, i, j, k) ? 1.0 : 0.0;
+    c64 v = in[e];
+    double kd = dir == 0 ? a.kx[i] : (dir == 1 ? a.ky[j] : a.kz[k]);
+    switch (op) {
+      case SOP_COPY: out[e] = v; break;
+      case SOP_MASK: out[e] = scal(v, m); break;
+      case SOP_IK_MASK: out[e] = scal(mk<double>(-kd * v.y, kd * v.x), m); break;
+      case SOP_NEGK2_MASK: {
+        double k2 = a.kx[i] * a.kx[i] + a.ky[j] * a.ky[j] + a.kz[k] * a.kz[k];
+        out[e] = scal(v, -k2 * m);
+        break;
+      }
+      case SOP_KERNEL_MASK: out[e] = scal(v, kern[e] * m); break;
+      case SOP_ACC_IK: out[e] = out[e] + mk<double>(-kd * v.y, kd * v.x); break;
+    }
+  }
+}
+
+static FullSpecArgs full_args(const Grid& g) {
+  FullSpecArgs a;
+  a.nkx = g.nkx; a.ny = g.ny; a.nz = g.nz; a.Kx = g.Kx; a.My = g.My; a.Mz = g.Mz;
+  a.kx = g.d_kx; a.ky = g.d_ky; a.kz = g.d_kz;
+  return a;
+}
+static int specop(const Grid& g, int op, int dir, const c64* in, const double* kern, c64* out,
+                  hipStream_t s) {
+  size_t n = g.nfull();
+  int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_specop, dim3(blocks), dim3(256), 0, s, full_args(g), op, dir, in, kern, out);
+  HY_HIP(hipGetLastError());
+  return HY_OK;
+}
+
+// Deterministic two-level sum of a real array.
+__global__ void k_sum_partial(const double* __restrict__ u, size_t n, double* __restrict__ part) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    acc += u[e];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void k_sum_final(const double* __restrict__ part, int np, double* __restrict__ out) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < np; e += blockDim.x) acc += part[e];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// -------------------------------------------------------- drop-in scratch
+// Eight full half-spectra, allocated on first use of a drop-in operator.
+static int grid_scratch(Grid& g, c64** s) {
+  const size_t need = sizeof(c64) * g.nfull() * 8 + sizeof(double) * 1024;
+  if (g.scratch_bytes < need) {
+    if (g.d_scratch) HY_HIP(hipFree(g.d_scratch));
+    g.d_scratch = nullptr;
+    if (hipMalloc(&g.d_scratch, need) != hipSuccess) {
+      (void)hipGetLastError();
+      g.scratch_bytes = 0;
+      return set_error(HY_ERR_OOM, "grid: cannot allocate drop-in scratch");
+    }
+    g.scratch_bytes = need;
+  }
+  for (int i = 0; i < 8; ++i) s[i] = reinterpret_cast<c64*>(g.d_scratch) + g.nfull() * i;
+  return HY_OK;
+}
+static double* grid_red(Grid& g) {
+  return reinterpret_cast<double*>(reinterpret_cast<c64*>(g.d_scratch) + g.nfull() * 8);
+}
+
+}  // namespace hy
+
+// ======================================================================= ABI
+using namespace hy;
+
+struct hy_grid_s {
+  Grid g;
+};
+
+extern "C" {
+
+int hy_init(int device) {
+  int n = 0;
+  HY_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return set_error(HY_ERR_INVALID, "hy_init: no such device");
+  HY_HIP(hipSetDevice(device));
+  return HY_OK;
+}
+const char* hy_last_error(void) { return g_last_error.c_str(); }
+int hy_version(void) { return 100; }
+int hy_device_sync(void) {
+  HY_HIP(hipDeviceSynchronize());
+  return HY_OK;
+}
+
+int hy_grid_create(int nx, int ny, int nz, double lx, double ly, double lz, int dealias,
+                   int precision, hy_grid* out) {
+  if (!out) return set_error(HY_ERR_INVALID, "hy_grid_create: null output");
+  auto* h = new hy_grid_s();
+  int rc = grid_build(h->g, nx, ny, nz, lx, ly, lz, dealias, precision);
+  if (rc != HY_OK) {
+    grid_free(h->g);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return HY_OK;
+}
+int hy_grid_destroy(hy_grid g) {
+  if (!g) return HY_OK;
+  grid_free(g->g);
+  delete g;
+  return HY_OK;
+}
+int hy_grid_query(hy_grid g, int* nkx, int* ndim, int* kept3, double* volume, double* kmax) {
+  if (!g) return set_error(HY_ERR_INVALID, "null grid");
+  if (nkx) *nkx = g->g.nkx;
+  if (ndim) *ndim = g->g.ndim;
+  if (kept3) { kept3[0] = g->g.Kx; kept3[1] = g->g.Ky; kept3[2] = g->g.Kz; }
+  if (volume) *volume = g->g.volume;
+  if (kmax) *kmax = g->g.kmax_dealiased;
+  return HY_OK;
+}
+int hy_grid_mask(hy_grid gh, double* h_mask) {
+  if (!gh || !h_mask) return set_error(HY_ERR_INVALID, "null argument");
+  const Grid& g = gh->g;
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nkx; ++i) {
+        int my = j <= g.ny / 2 ? j : j - g.ny, mz = k <= g.nz / 2 ? k : k - g.nz;
+        bool kept = i < g.Kx && std::abs(my) <= g.My && std::abs(mz) <= g.Mz;
+        h_mask[((size_t)k * g.ny + j) * g.nkx + i] = kept ? 1.0 : 0.0;
+      }
+  return HY_OK;
+}
+
+int hy_fft_r2c(hy_grid gh, const double* d_u, double* d_uhat, void* stream) {
+  if (!gh || !d_u || !d_uhat) return set_error(HY_ERR_INVALID, "null argument");
+  return fft_r2c_full(gh->g, d_u, reinterpret_cast<c64*>(d_uhat), (hipStream_t)stream);
+}
+
+int hy_fft_c2r(hy_grid gh, const double* d_uhat, double* d_u, void* stream) {
+  if (!gh || !d_u || !d_uhat) return set_error(HY_ERR_INVALID, "null argument");
+  Grid& g = gh->g;
+  c64* s[8];
+  HY_TRY(grid_scratch(g, s));
+  hipStream_t st = (hipStream_t)stream;
+  HY_HIP(hipMemcpyAsync(s[0], d_uhat, sizeof(c64) * g.nfull(), hipMemcpyDeviceToDevice, st));
+  return fft_c2r_full_inplace(g, s[0], d_u, st);
+}
+
+int hy_spec_grad_hat(hy_grid gh, const double* d_uhat, int idir, double* d_du, void* stream) {
+  if (!gh || !d_uhat || !d_du) return set_error(HY_ERR_INVALID, "null argument");
+  if (idir < 1 || idir > 3) return set_error(HY_ERR_INVALID, "spectral: bad direction");
+  Grid& g = gh->g;
+  c64* s[8];
+  HY_TRY(grid_scratch(g, s));
+  hipStream_t st = (hipStream_t)stream;
+  HY_TRY(specop(g, SOP_IK_MASK, idir - 1, reinterpret_cast<const c64*>(d_uhat), nullptr, s[0], st));
+  return fft_c2r_full_inplace(g, s[0], d_du, st);
+}
+
+int hy_spec_grad(hy_grid gh, const double* d_u, double* d_du3, void* stream) {
+  if (!gh || !d_u || !d_du3) return set_error(HY_ERR_INVALID, "null argument");
+  Grid& g = gh->g;
+  c64* s[8];
+  HY_TRY(grid_scratch(g, s));
+  hipStream_t st = (hipStream_t)stream;
+  HY_TRY(fft_r2c_full(g, d_u, s[1], st));
+  for (int d = 0; d < 3; ++d) {
+    double* dst = d_du3 + g.nreal() * d;
+    if (d >= g.ndim) {
+      HY_HIP(hipMemsetAsync(dst, 0, sizeof(double) * g.nreal(), st));
+      continue;
+    }
+    HY_TRY(specop(g, SOP_IK_MASK, d, s[1], nullptr, s[0], st));
+    HY_TRY(fft_c2r_full_inplace(g, s[0], dst, st));
+  }
+  return HY_OK;
+}
+
+int hy_spec_lap(hy_grid gh, const double* d_u, double* d_lu, void* stream) {
+  if (!gh || !d_u || !d_lu) return set_error(HY_ERR_INVALID, "null argument");
+  Grid& g = gh->g;
+  c64* s[8];
+  HY_TRY(grid_scratch(g, s));
+  hipStream_t st = (hipStream_t)stream;
+  HY_TRY(fft_r2c_full(g, d_u, s[1], st));
+  HY_TRY(specop(g, SOP_NEGK2_MASK, 0, s[1], nullptr, s[0], st));
+  return fft_c2r_full_inplace(g, s[0], d_lu, st);
+}
+
+int hy_spec_div(hy_grid gh, const double* d_v3, double* d_dv, void* stream) {
+  if (!gh || !d_v3 || !d_dv) return set_error(HY_ERR_INVALID, "null argument");
+  Grid& g = gh->g;
+  c64* s[8];
+  HY_TRY(grid_scratch(g, s));
+  hipStream_t st = (hipStream_t)stream;
+  HY_HIP(hipMemsetAsync(s[0], 0, sizeof(c64) * g.nfull(), st));
+  for (int d = 0; d < g.ndim; ++d) {
+    HY_TRY(fft_r2c_full(g, d_v3 + g.nreal() * d, s[1], st));
+    HY_TRY(specop(g, SOP_ACC_IK, d, s[1], nullptr, s[0], st));
+  }
+  HY_TRY(specop(g, SOP_MASK, 0, s[0], nullptr, s[0], st));
+  return fft_c2r_full_inplace(g, s[0], d_dv, st);
+}
+
+int hy_spec_convolve(hy_grid gh, const double* d_kernel, const double* d_uhat, double* d_cu,
+                     void* stream) {
+  if (!gh || !d_kernel || !d_uhat || !d_cu) return set_error(HY_ERR_INVALID, "null argument");
+  Grid& g = gh->g;
+  c64* s[8];
+  HY_TRY(grid_scratch(g, s));
+  hipStream_t st = (hipStream_t)stream;
+  HY_TRY(specop(g, SOP_KERNEL_MASK, 0, reinterpret_cast<const c64*>(d_uhat), d_kernel, s[0], st));
+  return fft_c2r_full_inplace(g, s[0], d_cu, st);
+}
+
+int hy_spec_dealias(hy_grid gh, double* d_u, void* stream) {
+  if (!gh || !d_u) return set_error(HY_ERR_INVALID, "null argument");
+  Grid& g = gh->g;
+  c64* s[8];
+  HY_TRY(grid_scratch(g, s));
+  hipStream_t st = (hipStream_t)stream;
+  HY_TRY(fft_r2c_full(g, d_u, s[0], st));
+  HY_TRY(specop(g, SOP_MASK, 0, s[0], nullptr, s[0], st));
+  return fft_c2r_full_inplace(g, s[0], d_u, st);
+}
+
+int hy_spec_mean(hy_grid gh, const double* d_u, double* h_mean, void* stream) {
+  if (!gh || !d_u || !h_mean) return set_error(HY_ERR_INVALID, "null argument");
+  Grid& g = gh->g;
+  c64* s[8];
+  HY_TRY(grid_scratch(g, s));
+  hipStream_t st = (hipStream_t)stream;
+  double* red = grid_red(g);
+  const int np = 512;
+  hipLaunchKernelGGL(k_sum_partial, dim3(np), dim3(256), 0, st, d_u, g.nreal(), red + 1);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, red + 1, np, red);
+  HY_HIP(hipGetLastError());
+  double sum = 0.0;
+  HY_HIP(hipMemcpyAsync(&sum, red, sizeof(double), hipMemcpyDeviceToHost, st));
+  HY_HIP(hipStreamSynchronize(st));
+  *h_mean = sum / (double)g.nreal();
+  return HY_OK;
+}
+
+int hy_spec_mode(hy_grid gh, const double* d_u, int mx, int my, int mz, double* h_re_im,
+                 void* stream) {
+  if (!gh || !d_u || !h_re_im) return set_error(HY_ERR_INVALID, "null argument");
+  Grid& g = gh->g;
+  if (mx < 0 || mx >= g.nkx) return set_error(HY_ERR_INVALID, "spec_mode: mx out of range");
+  c64* s[8];
+  HY_TRY(grid_scratch(g, s));
+  hipStream_t st = (hipStream_t)stream;
+  HY_TRY(fft_r2c_full(g, d_u, s[0], st));
+  int jy = ((my % g.ny) + g.ny) % g.ny, kz = ((mz % g.nz) + g.nz) % g.nz;
+  size_t idx = ((size_t)kz * g.ny + jy) * g.nkx + mx;
+  c64 v;
+  HY_HIP(hipMemcpyAsync(&v, s[0] + idx, sizeof(c64), hipMemcpyDeviceToHost, st));
+  HY_HIP(hipStreamSynchronize(st));
+  h_re_im[0] = v.x;
+  h_re_im[1] = v.y;
+  return HY_OK;
+}
+
+// stress_pi0 (stress.f90:60-83) evaluated spectrally: gu_ij = B(i k_j F(u_i) mask),
+// so each Pi component is the backward transform of the same linear combination.
+__global__ void k_pi0_combo(FullSpecArgs a, int comp, double eta, double xi,
+                            const c64* __restrict__ u0, const c64* __restrict__ u1,
+                            const c64* __restrict__ u2, int ndim, c64* __restrict__ out) {
+  size_t n = (size_t)a.nkx * a.ny * a.nz;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    int i = e % a.nkx, j = (e / a.nkx) % a.ny, k = e / ((size_t)a.nkx * a.ny);
+    double m = full_kept(a, i, j, k) ? 1.0 : 0.0;
+    double kv[3] = {a.kx[i], a.ky[j], a.kz[k]};
+    c64 uh[3] = {ndim > 0 ? u0[e] : mk<double>(0, 0), ndim > 1 ? u1[e] : mk<double>(0, 0),
+                 ndim > 2 ? u2[e] : mk<double>(0, 0)};
+    // g[i][j] = i k_j uhat_i (spectrum of du_i/dx_j)
+    auto g = [&](int ii, int jj) { c64 v = uh[ii]; return mk<double>(-kv[jj] * v.y, kv[jj] * v.x); };
+    c64 divu = g(0, 0) + g(1, 1) + g(2, 2);
+    c64 r;
+    if (comp < 3) {
+      c64 gii = g(comp, comp);
+      r = scal(scal(gii, 2.0) - scal(divu, 2.0 / 3.0), eta) + scal(divu, xi);
+    } else if (comp == 3) {
+      r = scal(g(0, 1) + g(1, 0), eta);
+    } else if (comp == 4) {
+      r = scal(g(0, 2) + g(2, 0), eta);
+    } else {
+      r = scal(g(1, 2) + g(2, 1), eta);
+    }
+    out[e] = scal(r, m);
+  }
+}
+
+int hy_stress_pi0(hy_grid gh, double eta, double xi, const double* d_u3, double* d_pi6, void* stream) {
+  if (!gh || !d_u3 || !d_pi6) return set_error(HY_ERR_INVALID, "null argument");
+  Grid& g = gh->g;
+  c64* s[8];
+  HY_TRY(grid_scratch(g, s));
+  hipStream_t st = (hipStream_t)stream;
+  for (int d = 0; d < g.ndim; ++d) HY_TRY(fft_r2c_full(g, d_u3 + g.nreal() * d, s[1 + d], st));
+  size_t n = g.nfull();
+  int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+  for (int c = 0; c < 6; ++c) {
+    hipLaunchKernelGGL(k_pi0_combo, dim3(blocks), dim3(256), 0, st, full_args(g), c, eta, xi, s[1],
+                       s[2], s[3], g.ndim, s[0]);
+    HY_HIP(hipGetLastError());
+    HY_TRY(fft_c2r_full_inplace(g, s[0], d_pi6 + g.nreal() * c, st));
+  }
+  return HY_OK;
+}
+
+int hy_stress_div(hy_grid gh, const double* d_pi6, double* d_f3, void* stream) {
+  if (!gh || !d_pi6 || !d_f3) return set_error(HY_ERR_INVALID, "null argument");
+  Grid& g = gh->g;
+  c64* s[8];
+  HY_TRY(grid_scratch(g, s));
+  hipStream_t st = (hipStream_t)stream;
+  static const int idx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};  // stress.f90:92
+  for (int c = 0; c < 6; ++c) HY_TRY(fft_r2c_full(g, d_pi6 + g.nreal() * c, s[1 + c], st));
+  for (int i = 0; i < 3; ++i) {
+    double* dst = d_f3 + g.nreal() * i;
+    if (i >= g.ndim) {
+      HY_HIP(hipMemsetAsync(dst, 0, sizeof(double) * g.nreal(), st));
+      continue;
+    }
+    HY_HIP(hipMemsetAsync(s[0], 0, sizeof(c64) * g.nfull(), st));
+    for (int j = 0; j < g.ndim; ++j) HY_TRY(specop(g, SOP_ACC_IK, j, s[1 + idx[i][j]], nullptr, s[0], st));
+    HY_TRY(specop(g, SOP_MASK, 0, s[0], nullptr, s[0], st));
+    HY_TRY(fft_c2r_full_inplace(g, s[0], dst, st));
+  }
+  return HY_OK;
+}
+
+}  // extern "C"

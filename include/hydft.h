@@ -1,0 +1,158 @@
+/*
+ * libhydft — MI355X (gfx950) HIP implementation of HyDFT's 3-D pseudo-spectral
+ * right-hand side and low-storage RK3 time loop, behind a plain C ABI.
+ *
+ * Conventions (those of the reference, abdoudiaw/HyDFT):
+ *  - Real fields are (nx, ny, nz) arrays with x fastest (Fortran column-major),
+ *    index i + nx*(j + ny*k).  Half spectra are (nkx = nx/2+1, ny, nz) complex
+ *    (interleaved re, im doubles), x fastest: FFTW's r2c_3d(nz, ny, nx) layout.
+ *  - Forward transforms are normalised by 1/(nx*ny*nz); backward are not and
+ *    preserve their input (src/core/fft.f90:52-69).
+ *  - Vector fields are component-major: component c at offset c*nx*ny*nz.
+ *  - Stress components are (xx, yy, zz, xy, xz, yz) (src/closure/stress.f90:11).
+ *  - Every call returns HY_OK (0) or an error code; hy_last_error() gives the
+ *    message.  The Fortran binding maps a nonzero code to fatal() (error stop 1),
+ *    as the reference does (src/core/kinds.f90:26-30).
+ *  - Device-pointer entry points (suffix-free `d_` arguments) are stream
+ *    ordered on `stream` (a hipStream_t, NULL = default stream).  Host-pointer
+ *    entry points (`h_` arguments) synchronise before returning.
+ *  - Handles are not thread-safe; one model per host thread at a time.
+ */
+#ifndef HYDFT_H
+#define HYDFT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HY_OK 0
+#define HY_ERR_INVALID 1        /* invalid argument or grid (grid.f90:45-48)            */
+#define HY_ERR_NONPOSITIVE 2    /* density became non-positive (model.f90:91)          */
+#define HY_ERR_OOM 3            /* device allocation failed                            */
+#define HY_ERR_HIP 4            /* HIP runtime error                                   */
+#define HY_ERR_UNKNOWN_KIND 5   /* unknown term / closure / potential name             */
+#define HY_ERR_UNSUPPORTED 6    /* grid size outside {4..256} of the form 2^a 3^b      */
+#define HY_ERR_NO_DT 7          /* cannot determine a time step (model.f90:180)        */
+#define HY_ERR_BUSY 8           /* serving: no slot or HBM budget for the request      */
+
+#define HY_PREC_FP64 0          /* everything in FP64                                  */
+#define HY_PREC_MIXED 1         /* FP32 transforms and functional kernels, FP64 state  */
+
+typedef struct hy_grid_s* hy_grid;
+typedef struct hy_model_s* hy_model;
+
+/* ---- library ----------------------------------------------------------- */
+int hy_init(int device);
+const char* hy_last_error(void);
+int hy_version(void);
+int hy_device_sync(void);
+
+/* ---- grid_t + fft_t (src/core/grid.f90:34-113, src/core/fft.f90:36-49) -- */
+int hy_grid_create(int nx, int ny, int nz, double lx, double ly, double lz, int dealias,
+                   int precision, hy_grid* out);
+int hy_grid_destroy(hy_grid g);
+/* nkx, ndim, kept box (Kx, Ky, Kz), volume, kmax_dealiased (grid.f90:15-30) */
+int hy_grid_query(hy_grid g, int* nkx, int* ndim, int* kept3, double* volume,
+                  double* kmax_dealiased);
+/* the 2/3-rule mask on the half spectrum, (nkx, ny, nz) doubles (grid.f90:89-112) */
+int hy_grid_mask(hy_grid g, double* h_mask);
+
+/* ---- fft_t (src/core/fft.f90:52-69), FP64 device buffers ---------------- */
+int hy_fft_r2c(hy_grid g, const double* d_u, double* d_uhat, void* stream);
+int hy_fft_c2r(hy_grid g, const double* d_uhat, double* d_u, void* stream);
+
+/* ---- spec_t (src/core/spectral.f90:67-188), FP64 device buffers --------- */
+int hy_spec_grad_hat(hy_grid g, const double* d_uhat, int idir, double* d_du, void* stream);
+int hy_spec_grad(hy_grid g, const double* d_u, double* d_du3, void* stream);
+int hy_spec_lap(hy_grid g, const double* d_u, double* d_lu, void* stream);
+int hy_spec_div(hy_grid g, const double* d_v3, double* d_dv, void* stream);
+int hy_spec_convolve(hy_grid g, const double* d_kernel, const double* d_uhat, double* d_cu,
+                     void* stream);
+int hy_spec_dealias(hy_grid g, double* d_u, void* stream);
+int hy_spec_mean(hy_grid g, const double* d_u, double* h_mean, void* stream);
+int hy_spec_mode(hy_grid g, const double* d_u, int mx, int my, int mz, double* h_re_im,
+                 void* stream);
+
+/* ---- stress_t (src/closure/stress.f90:60-105), code-unit eta, xi -------- */
+int hy_stress_pi0(hy_grid g, double eta, double xi, const double* d_u3, double* d_pi6,
+                  void* stream);
+int hy_stress_div(hy_grid g, const double* d_pi6, double* d_f3, void* stream);
+
+/* ---- free-energy terms (src/functional/*.f90) ---------------------------- */
+#define HY_TERM_IDEAL 0     /* fe_ideal_classical.f90 */
+#define HY_TERM_TF 1        /* fe_kinetic_tfk.f90 fe_tf_t */
+#define HY_TERM_GRADIENT 2  /* fe_kinetic_tfk.f90 fe_gradient_t */
+#define HY_TERM_HARTREE 3   /* fe_hartree.f90 */
+#define HY_TERM_RY 4        /* fe_correlation_ry.f90 */
+
+typedef struct {
+  int kind;                 /* HY_TERM_*                                       */
+  double n0, beta;          /* fe_term_t%n0, %beta (free_energy.f90:23-25)     */
+  double e2, kappa;         /* hartree / ry                                    */
+  double cn, alpha0, cg;    /* tf / gradient (units.f90:31, species.f90:127)   */
+  int subtract_mean_field;  /* ry (species.f90:136)                            */
+  int ntab;                 /* ry: radial table size                           */
+  const double* k_tab;      /* ry: host table k (increasing, > 0)              */
+  const double* c_tab;      /* ry: host table c(k) (structure_source.f90:18-26)*/
+} hy_term_desc;
+
+/* ---- model_t (src/model/model.f90) -------------------------------------- */
+#define HY_CLOSURE_NEWTONIAN 0
+#define HY_CLOSURE_MAXWELL 1
+#define HY_VEXT_NONE 0
+#define HY_VEXT_MODE 1
+#define HY_VEXT_DRIVEN 2
+#define HY_VEXT_GAUSSIAN 3
+
+typedef struct {
+  int nx, ny, nz;
+  double lx, ly, lz;
+  int dealias;              /* &grid dealias (params.f90:21)                    */
+  int precision;            /* HY_PREC_*                                        */
+  double mass, n0;
+  int nterms;
+  const hy_term_desc* terms;
+  int closure;              /* HY_CLOSURE_* (stress.f90:43-52)                  */
+  double eta, xi, tau;      /* stress_t%eta, %xi in code units, tau (stress.f90:53-54) */
+  double eta_l;             /* transport_t%eta_l for compute_dt (model.f90:165,170) */
+  int vext_kind;            /* HY_VEXT_* (external_potential.f90:49-58)         */
+  double vext_amp;
+  int vext_mode[3];
+  double vext_omega, vext_width, vext_ramp;
+  double dt, cfl;           /* &time dt (> 0 fixed) and cfl (params.f90:45-46)   */
+} hy_model_desc;
+
+int hy_model_create(const hy_model_desc* desc, hy_model* out);
+int hy_model_destroy(hy_model m);
+/* state in the reference layout; projected onto the kept band on entry */
+int hy_model_set_state(hy_model m, const double* h_n, const double* h_mom3, const double* h_pi6);
+int hy_model_get_state(hy_model m, double* h_n, double* h_mom3, double* h_pi6);
+int hy_model_set_time(hy_model m, double t, int step);
+int hy_model_get_time(hy_model m, double* t, double* dt, int* step);
+int hy_model_set_dt(hy_model m, double dt);
+/* model_compute_dt (model.f90:150-183) */
+int hy_model_compute_dt(hy_model m, double* h_dt);
+/* model_advance x nsteps (model.f90:186-227).  dt_every = 0 keeps the current
+   dt; k > 0 calls compute_dt before steps 0, k, 2k, ... of this call (the
+   driver cadence of src/hydft.f90:42-43 with k = 20).  tend > 0 clamps the
+   last step to tend and stops there (src/hydft.f90:41,44).  When probe3 is not
+   NULL, mode probe3 of n/n0 and mom_x (diagnostics.f90:70-77) is recorded
+   after every step into h_probe (6 doubles per step: t, Re, Im dn_k/n0,
+   Re, Im mom_k, dt).  Returns the number of steps taken in *h_taken. */
+int hy_model_advance(hy_model m, int nsteps, int dt_every, double tend, const int* probe3,
+                     double* h_probe, int* h_taken);
+/* model_rhs (model.f90:85-146) at time t from the current state */
+int hy_model_rhs(hy_model m, double t, double* h_dn, double* h_dmom3, double* h_dpi6);
+/* functional_mu (functional.f90:56-67) of a given real density */
+int hy_model_mu(hy_model m, const double* h_n, double* h_mu);
+/* diag_budgets (diagnostics.f90:41-60): mass, mom x/y/z, E_kin, F */
+int hy_model_budgets(hy_model m, double* h_out6);
+/* spec_mode of n/n0 and mom_x at (mx, my, mz) (diagnostics.f90:70-77) */
+int hy_model_probe(hy_model m, int mx, int my, int mz, double* h_out4);
+/* functional_kernel K(k) (functional.f90:64-72) */
+int hy_model_kernel(hy_model m, double k, double* h_K);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HYDFT_H */
