@@ -105,6 +105,7 @@ _SIGS = {
     "hy_model_budgets": [_P, _P],
     "hy_model_probe": [_P, _I, _I, _I, _P],
     "hy_model_kernel": [_P, _D, _PD],
+    "hy_fd_eval": [_I, _I, _P, _P],
 }
 
 
@@ -122,8 +123,13 @@ class _Lib:
         self.lib = C.CDLL(str(path))
         self.lib.hy_last_error.restype = C.c_char_p
         self.lib.hy_last_error.argtypes = []
+        self.missing = []
         for name, args in _SIGS.items():
-            fn = getattr(self.lib, name)
+            try:
+                fn = getattr(self.lib, name)
+            except AttributeError:
+                self.missing.append(name)
+                continue
             fn.argtypes = args
             fn.restype = C.c_int
 

@@ -1,0 +1,125 @@
+// Fermi-Dirac integrals I_p(alpha), p = -1/2, 1/2, 3/2, and alpha-derivatives of
+// I_{-1/2}, with the inverse of I_{1/2} (fermi_dirac.f90).  A 6-column table on
+// alpha in [-30, 300], h = 0.05, built once on the host by 20-point
+// Gauss-Legendre panels, evaluated by cubic Hermite interpolation using the
+// exact derivative relations; Sommerfeld / classical expansions outside.
+// The same __host__ __device__ code serves the host (units, kernels) and the
+// per-point electron functional on the GPU.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace hy {
+namespace fd {
+
+constexpr double AMIN = -30.0, AMAX = 300.0, H = 0.05;
+constexpr int NTAB = 6601;  // nint((AMAX - AMIN)/H) + 1
+constexpr int NC = 6;       // columns: I-1/2, I1/2, I3/2, I'-1/2, I''-1/2, I'''-1/2
+constexpr double PI = 3.14159265358979323846264338327950288;
+constexpr double SQRTPI = 1.77245385090551602729816748334114518;
+
+// Host-built table, tab[i*NC + c] (hy_model.hip).
+const double* host_table();
+
+__host__ __device__ inline int node(double a) {
+  int i = (int)((a - AMIN) / H);
+  return i < 0 ? 0 : (i > NTAB - 2 ? NTAB - 2 : i);
+}
+
+// cubic Hermite of column `col` with derivative values d0, d1 at the nodes i, i+1
+__host__ __device__ inline double hermite(const double* tab, int col, double a, int i, double d0, double d1) {
+  double t = (a - (AMIN + i * H)) / H;
+  double t2 = t * t, t3 = t2 * t;
+  double h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + t, h01 = -2 * t3 + 3 * t2, h11 = t3 - t2;
+  double y0 = tab[i * NC + col], y1 = tab[(i + 1) * NC + col];
+  return h00 * y0 + h10 * H * d0 + h01 * y1 + h11 * H * d1;
+}
+
+// m-th alpha-derivative of I_p from the Sommerfeld expansion (alpha >> 1)
+__host__ __device__ inline double sommerfeld(double p, double a, int m) {
+  double c[3] = {1.0, (PI * PI / 6.0) * p * (p + 1.0),
+                 (7.0 * PI * PI * PI * PI / 360.0) * (p + 1.0) * p * (p - 1.0) * (p - 2.0)};
+  double v = 0.0;
+  for (int j = 0; j < 3; ++j) {
+    double ex = p + 1.0 - 2 * j;
+    double coef = c[j] / (p + 1.0);
+    for (int l = 0; l < m; ++l) coef *= (ex - l);
+    v += coef * pow(a, ex - m);
+  }
+  return v;
+}
+
+// m-th derivative of I_p from the classical expansion (alpha << -1); gp1 = Gamma(p+1)
+__host__ __device__ inline double classical(double p, double a, int m, double gp1) {
+  double v = 0.0, sgn = 1.0;
+  for (int j = 1; j <= 12; ++j) {
+    v += sgn * exp(j * a) * pow((double)j, m - p - 1.0);
+    sgn = -sgn;
+  }
+  return v * gp1;
+}
+
+__host__ __device__ inline double im12(const double* tab, double a) {
+  if (a > AMAX) return sommerfeld(-0.5, a, 0);
+  if (a < AMIN) return classical(-0.5, a, 0, SQRTPI);
+  int i = node(a);
+  return hermite(tab, 0, a, i, tab[i * NC + 3], tab[(i + 1) * NC + 3]);
+}
+__host__ __device__ inline double ip12(const double* tab, double a) {
+  if (a > AMAX) return sommerfeld(0.5, a, 0);
+  if (a < AMIN) return classical(0.5, a, 0, 0.5 * SQRTPI);
+  int i = node(a);
+  return hermite(tab, 1, a, i, 0.5 * tab[i * NC + 0], 0.5 * tab[(i + 1) * NC + 0]);
+}
+__host__ __device__ inline double ip32(const double* tab, double a) {
+  if (a > AMAX) return sommerfeld(1.5, a, 0);
+  if (a < AMIN) return classical(1.5, a, 0, 0.75 * SQRTPI);
+  int i = node(a);
+  return hermite(tab, 2, a, i, 1.5 * tab[i * NC + 1], 1.5 * tab[(i + 1) * NC + 1]);
+}
+__host__ __device__ inline double dim12(const double* tab, double a) {
+  if (a > AMAX) return sommerfeld(-0.5, a, 1);
+  if (a < AMIN) return classical(-0.5, a, 1, SQRTPI);
+  int i = node(a);
+  return hermite(tab, 3, a, i, tab[i * NC + 4], tab[(i + 1) * NC + 4]);
+}
+__host__ __device__ inline double d2im12(const double* tab, double a) {
+  if (a > AMAX) return sommerfeld(-0.5, a, 2);
+  if (a < AMIN) return classical(-0.5, a, 2, SQRTPI);
+  int i = node(a);
+  return hermite(tab, 4, a, i, tab[i * NC + 5], tab[(i + 1) * NC + 5]);
+}
+// xi = I'_{-1/2}/I_{-1/2}^2 and its alpha-derivative (fermi_dirac.f90:225-238)
+__host__ __device__ inline double xi(const double* tab, double a) {
+  double im = im12(tab, a);
+  return dim12(tab, a) / (im * im);
+}
+__host__ __device__ inline double dxi(const double* tab, double a) {
+  double im = im12(tab, a), d1 = dim12(tab, a), d2 = d2im12(tab, a);
+  return (d2 * im - 2.0 * d1 * d1) / (im * im * im);
+}
+
+// alpha with I_{1/2}(alpha) = y > 0: safeguarded Newton (fermi_dirac.f90:241-267)
+__host__ __device__ inline double alpha_from_ip12(const double* tab, double y) {
+  const double huge = 1.7976931348623157e308;
+  double alpha = fmax(log(2.0 * y / SQRTPI), pow(1.5 * y, 2.0 / 3.0) - 1.0);
+  if (y < 1.0) alpha = log(2.0 * y / SQRTPI);
+  double lo = -huge, hi = huge;
+  for (int it = 0; it < 100; ++it) {
+    double f = ip12(tab, alpha) - y;
+    if (f > 0.0) hi = fmin(hi, alpha);
+    else lo = fmax(lo, alpha);
+    double df = 0.5 * im12(tab, alpha);
+    double step = f / df;
+    if (fabs(step) < 1.0e-14 * fmax(1.0, fabs(alpha))) break;
+    alpha = alpha - step;
+    if (alpha <= lo || alpha >= hi) {
+      if (lo > -huge && hi < huge) alpha = 0.5 * (lo + hi);
+    }
+  }
+  return alpha;
+}
+
+}  // namespace fd
+}  // namespace hy

@@ -501,6 +501,30 @@ static double* grid_red(Grid& g) {
   return reinterpret_cast<double*>(reinterpret_cast<c64*>(g.d_scratch) + g.nfull() * 8);
 }
 
+static int spec_grad_impl(Grid& g, const double* d_u, double* d_du3, hipStream_t st) {
+  c64* s[8];
+  HY_TRY(grid_scratch(g, s));
+  HY_TRY(fft_r2c_full(g, d_u, s[1], st));
+  for (int d = 0; d < 3; ++d) {
+    double* dst = d_du3 + g.nreal() * d;
+    if (d >= g.ndim) {
+      HY_HIP(hipMemsetAsync(dst, 0, sizeof(double) * g.nreal(), st));
+      continue;
+    }
+    HY_TRY(specop(g, SOP_IK_MASK, d, s[1], nullptr, s[0], st));
+    HY_TRY(fft_c2r_full_inplace(g, s[0], dst, st));
+  }
+  return HY_OK;
+}
+
+// entry points for the model translation unit
+int fft_r2c_full_api(Grid& g, const double* u, c64* uhat, hipStream_t s) { return fft_r2c_full(g, u, uhat, s); }
+// destroys its input spectrum
+int fft_c2r_full_api(Grid& g, const c64* uhat, double* u, hipStream_t s) {
+  return fft_c2r_full_inplace(g, const_cast<c64*>(uhat), u, s);
+}
+int spec_grad_api(Grid& g, const double* u, double* du3, hipStream_t s) { return spec_grad_impl(g, u, du3, s); }
+
 }  // namespace hy
 
 // ======================================================================= ABI
@@ -595,21 +619,7 @@ int hy_spec_grad_hat(hy_grid gh, const double* d_uhat, int idir, double* d_du, v
 
 int hy_spec_grad(hy_grid gh, const double* d_u, double* d_du3, void* stream) {
   if (!gh || !d_u || !d_du3) return set_error(HY_ERR_INVALID, "null argument");
-  Grid& g = gh->g;
-  c64* s[8];
-  HY_TRY(grid_scratch(g, s));
-  hipStream_t st = (hipStream_t)stream;
-  HY_TRY(fft_r2c_full(g, d_u, s[1], st));
-  for (int d = 0; d < 3; ++d) {
-    double* dst = d_du3 + g.nreal() * d;
-    if (d >= g.ndim) {
-      HY_HIP(hipMemsetAsync(dst, 0, sizeof(double) * g.nreal(), st));
-      continue;
-    }
-    HY_TRY(specop(g, SOP_IK_MASK, d, s[1], nullptr, s[0], st));
-    HY_TRY(fft_c2r_full_inplace(g, s[0], dst, st));
-  }
-  return HY_OK;
+  return spec_grad_impl(gh->g, d_u, d_du3, (hipStream_t)stream);
 }
 
 int hy_spec_lap(hy_grid gh, const double* d_u, double* d_lu, void* stream) {

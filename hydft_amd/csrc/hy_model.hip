@@ -1,0 +1,1597 @@
+// The time-dependent model (model.f90) on the GPU with a spectral state.
+//
+// State: the masked half spectra of n, mom(3) and, for the Maxwell closure,
+// Pi(6), stored only on the kept 2/3-rule box (kx index < Kx, |my| <= My,
+// |mz| <= Mz) in FP64.  The reference dealiases the state after every stage,
+// so its real-space state is always band limited and equals the backward
+// transform of this spectral state.
+//
+// One RK3 stage:
+//   A  state -> x-lines (z, y inverse passes) -> x kernel A: n, u, mu_loc, flux
+//      -> y, z forward passes
+//   K1 mu_hat = F(mu_loc) + K(k) n_hat'   (+ pending Maxwell relaxation)
+//   B  mu_hat, Pi_hat -> x-lines -> x kernel B: n (f_ext - grad mu), -u.grad Pi
+//      -> y, z forward passes
+//   K2 R_hat assembly (continuity, flux divergence, stress divergence) and the
+//      low-storage RK3 update S += dt (a R + b R_old), R_old = R.
+// Real fields never leave LDS/registers: each x kernel inverse-transforms its
+// inputs along x, does the pointwise physics and forward-transforms the results.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "hy_fd.h"
+#include "hy_internal.h"
+
+namespace hy {
+
+// ------------------------------------------------------------ FD table (host)
+static std::vector<double> g_fdtab;
+static std::once_flag g_fdonce;
+
+static void gauss_legendre(int n, double* x, double* w) {  // quadrature.f90:17-38
+  for (int i = 1; i <= (n + 1) / 2; ++i) {
+    double z = std::cos(fd::PI * (i - 0.25) / (n + 0.5)), pp = 0, z1;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1.0, p2 = 0.0, p3;
+      for (int j = 1; j <= n; ++j) {
+        p3 = p2; p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p1 / pp;
+      if (std::fabs(z - z1) < 1.0e-15) break;
+    }
+    x[i - 1] = -z; x[n - i] = z;
+    w[i - 1] = 2.0 / ((1.0 - z * z) * pp * pp); w[n - i] = w[i - 1];
+  }
+}
+static void fermi_derivs(double y, double& f, double& p1, double& p2, double& p3) {
+  if (y > 0.0) { double g = std::exp(-y); f = g / (1.0 + g); }
+  else { double g = std::exp(y); f = 1.0 / (1.0 + g); }
+  p1 = f * (1.0 - f);
+  p2 = p1 * (1.0 - 2.0 * f);
+  p3 = p1 * (1.0 - 6.0 * f + 6.0 * f * f);
+}
+static void fd_build() {  // fermi_dirac.f90:30-74
+  const int ngl = 20;
+  double xg[ngl], wg[ngl];
+  gauss_legendre(ngl, xg, wg);
+  g_fdtab.assign((size_t)fd::NTAB * fd::NC, 0.0);
+  for (int i = 0; i < fd::NTAB; ++i) {
+    double alpha = fd::AMIN + i * fd::H;
+    double xmax = std::max(alpha, 0.0) + 45.0;
+    double s[6] = {0, 0, 0, 0, 0, 0}, f, p1, p2, p3;
+    for (int q = 0; q < ngl; ++q) {
+      double t = 0.5 * (xg[q] + 1.0), x = t * t;
+      fermi_derivs(x - alpha, f, p1, p2, p3);
+      double e = wg[q] * 0.5 * 2.0;
+      s[0] += e * f; s[1] += e * t * t * f; s[2] += e * std::pow(t, 4) * f;
+      s[3] += e * p1; s[4] += e * p2; s[5] += e * p3;
+    }
+    int npan = (int)std::ceil(xmax - 1.0);
+    for (int ip = 1; ip <= npan; ++ip) {
+      double a = 1.0 + (ip - 1), b = std::min(a + 1.0, xmax);
+      double xm = 0.5 * (a + b), xr = 0.5 * (b - a);
+      for (int q = 0; q < ngl; ++q) {
+        double x = xm + xr * xg[q];
+        fermi_derivs(x - alpha, f, p1, p2, p3);
+        double e = wg[q] * xr, t = 1.0 / std::sqrt(x), sx = std::sqrt(x);
+        s[0] += e * t * f; s[1] += e * sx * f; s[2] += e * x * sx * f;
+        s[3] += e * t * p1; s[4] += e * t * p2; s[5] += e * t * p3;
+      }
+    }
+    for (int c = 0; c < 6; ++c) g_fdtab[(size_t)i * fd::NC + c] = s[c];
+  }
+}
+const double* fd::host_table() {
+  std::call_once(g_fdonce, fd_build);
+  return g_fdtab.data();
+}
+
+// ------------------------------------------------------------ device control
+struct Ctl {
+  double t, dt, ts;          // model time, step size, current stage time
+  double relax_fac, relax_phi;
+  double nf[3];              // mean(n f_ext,i) of the current stage
+  double umax;
+  int step;
+  int nonpos;                // set when a non-positive density is seen
+  int nodt;                  // compute_dt could not find a time step
+};
+
+__constant__ double c_rk_a[3] = {32.0 / 60.0, 25.0 / 60.0, 45.0 / 60.0};  // model.f90:51-52
+__constant__ double c_rk_b[3] = {0.0, -17.0 / 60.0, -25.0 / 60.0};
+static const double h_rk_a[3] = {32.0 / 60.0, 25.0 / 60.0, 45.0 / 60.0};
+static const double h_rk_b[3] = {0.0, -17.0 / 60.0, -25.0 / 60.0};
+
+enum CtlOp { CTL_STAGE_END = 0, CTL_STEP_END = 1, CTL_STEP_BEGIN = 2 };
+__global__ void k_ctl(Ctl* c, int op, int s, double tau, int evolve, double tend) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (op == CTL_STEP_BEGIN) {
+    if (tend > 0.0 && c->t + c->dt > tend) c->dt = tend - c->t;  // src/hydft.f90:44
+    c->ts = c->t;
+  } else if (op == CTL_STAGE_END) {
+    double dts = (c_rk_a[s] + c_rk_b[s]) * c->dt;
+    c->ts = c->ts + dts;
+    if (evolve) {
+      double fac = exp(-dts / tau);
+      c->relax_fac = fac;
+      c->relax_phi = (tau / dts) * (1.0 - fac);
+    }
+  } else if (op == CTL_STEP_END) {
+    c->t = c->t + c->dt;
+    c->step += 1;
+    c->ts = c->t;
+  }
+}
+
+// ------------------------------------------------------------ x-line inputs
+struct XIn {
+  const void* p;   // (Kx, nlines) spectral x-lines
+  int is64;        // element type c64 (1) or compute type (0)
+  int mul;         // 0 none, 1 i kx, 2 -kx^2
+  double k0off;    // added to the (kx=0, line 0) entry (1-D first pass)
+};
+enum XMul { XM_NONE = 0, XM_IK = 1, XM_NEGK2 = 2 };
+
+template <class T>
+__device__ inline cpx<T> xin_load(const XIn& in, size_t line, int Kx, int kk, const double* kx) {
+  cpx<T> v = in.is64 ? cvt<T>(reinterpret_cast<const c64*>(in.p)[line * Kx + kk])
+                     : reinterpret_cast<const cpx<T>*>(in.p)[line * Kx + kk];
+  if (in.k0off != 0.0 && kk == 0 && line == 0) v.x = (T)((double)v.x + in.k0off);
+  if (in.mul == XM_IK) {
+    double k = kx[kk];
+    v = mk<T>((T)(-k * (double)v.y), (T)(k * (double)v.x));
+  } else if (in.mul == XM_NEGK2) {
+    double k2 = kx[kk] * kx[kk];
+    v = mk<T>((T)(-k2 * (double)v.x), (T)(-k2 * (double)v.y));
+  }
+  return v;
+}
+
+// Inverse x transform of two spectral x-lines packed as a + i b; afterwards
+// buf[x] = a(x) + i b(x).  DC and Nyquist imaginary parts are dropped, as a
+// c2r transform does.
+template <class T, int NX>
+__device__ inline void x_inv_pair(cpx<T>* buf, const cpx<T>* tw, const XIn* A, const XIn* B,
+                                  size_t line, int Kx, const double* kx, int lane) {
+  __syncthreads();
+  for (int kk = lane; kk <= NX / 2; kk += 64) {
+    cpx<T> a = mk<T>(0, 0), b = mk<T>(0, 0);
+    if (kk < Kx) {
+      if (A) a = xin_load<T>(*A, line, Kx, kk, kx);
+      if (B) b = xin_load<T>(*B, line, Kx, kk, kx);
+    }
+    if (kk == 0 || 2 * kk == NX) { a.y = 0; b.y = 0; }
+    buf[kk] = mk<T>(a.x - b.y, a.y + b.x);
+    if (kk > 0 && 2 * kk < NX) buf[NX - kk] = mk<T>(a.x + b.y, -a.y + b.x);  // conj(a) + i conj(b)
+  }
+  __syncthreads();
+  fft_lds<T, NX, 1, 64, true>(buf, tw, lane);
+}
+
+// Forward x transform of two real lines va + i vb; stores the first Kx
+// coefficients of each, scaled by 1/NX, to outA / outB (compute type).
+template <class T, int NX, int PTS>
+__device__ inline void x_fwd_pair(cpx<T>* buf, const cpx<T>* tw, const double* va, const double* vb,
+                                  void* outA, void* outB, size_t line, int Kx, int lane) {
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PTS; ++p) {
+    int x = lane + 64 * p;
+    if (x < NX) buf[x] = mk<T>((T)va[p], vb ? (T)vb[p] : T(0));
+  }
+  __syncthreads();
+  fft_lds<T, NX, 1, 64, false>(buf, tw, lane);
+  const T sc = (T)(0.5 / NX);
+  cpx<T>* oA = reinterpret_cast<cpx<T>*>(outA);
+  cpx<T>* oB = reinterpret_cast<cpx<T>*>(outB);
+  for (int k = lane; k < Kx; k += 64) {
+    cpx<T> Z = buf[k], Zc = conjc(buf[(NX - k) % NX]);
+    oA[line * Kx + k] = scal(Z + Zc, sc);
+    if (oB) {
+      cpx<T> D = Z - Zc;
+      oB[line * Kx + k] = mk<T>(D.y * sc, -D.x * sc);
+    }
+  }
+}
+
+template <class T, int NX, int PTS>
+__device__ inline void x_get(const cpx<T>* buf, double* a, double* b, int lane) {
+#pragma unroll
+  for (int p = 0; p < PTS; ++p) {
+    int x = lane + 64 * p;
+    cpx<T> v = x < NX ? buf[x] : mk<T>(0, 0);
+    if (a) a[p] = (double)v.x;
+    if (b) b[p] = (double)v.y;
+  }
+}
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// ---------------------------------------------------------- physics args
+enum OutA { OA_MU = 0, OA_U1, OA_U2, OA_U3, OA_F11, OA_F12, OA_F13, OA_F22, OA_F23, OA_F33 };
+enum OutB { OB_H1 = 0, OB_H2, OB_H3, OB_ADV1 };  // ADV1..ADV6 = 3..8
+
+struct PhysArgs {
+  int nd, Kx, ny, nz;
+  double dx, dy, dz;
+  const double* kx;
+  double n0, mass, beta, cn, alpha0, cg;
+  int ideal, tf, grad;
+  const double* fdtab;
+  Ctl* ctl;
+  // phase A inputs
+  XIn n, m[3], gx, gy, gz, lap;
+  // phase B inputs
+  XIn mux, muy, muz;
+  XIn px[6], py[6], pz[6];
+  int evolve;
+  // external force
+  int vext_kind, vext_td;
+  double amp, ramp, omega, kvec[3];
+  const double* fext;  // gaussian: 3 real fields
+  double* red;         // per-line partial sums
+  // outputs
+  int nout;
+  int code[10];
+  void* dst[10];
+};
+
+template <class T, int NX>
+__global__ __launch_bounds__(64) void k_phaseA(PhysArgs a, const cpx<T>* __restrict__ twg) {
+  constexpr int PTS = NX >= 64 ? NX / 64 : 1;
+  __shared__ cpx<T> buf[NX];
+  __shared__ cpx<T> tw[NX];
+  const int lane = threadIdx.x;
+  const size_t line = blockIdx.x;
+  for (int j = lane; j < NX; j += 64) tw[j] = twg[j];
+  double dn[PTS], m0[PTS], m1[PTS], m2[PTS];
+  x_inv_pair<T, NX>(buf, tw, &a.n, &a.m[0], line, a.Kx, a.kx, lane);
+  x_get<T, NX, PTS>(buf, dn, m0, lane);
+  if (a.nd >= 2) {
+    x_inv_pair<T, NX>(buf, tw, &a.m[1], a.nd >= 3 ? &a.m[2] : nullptr, line, a.Kx, a.kx, lane);
+    x_get<T, NX, PTS>(buf, m1, m2, lane);
+  } else {
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) { m1[p] = 0.0; m2[p] = 0.0; }
+  }
+  double mu[PTS], u0[PTS], u1[PTS], u2[PTS];
+  double gx[PTS], gy[PTS], gz[PTS], lap[PTS];
+  if (a.grad) {
+    x_inv_pair<T, NX>(buf, tw, &a.gx, a.nd >= 2 ? &a.gy : nullptr, line, a.Kx, a.kx, lane);
+    x_get<T, NX, PTS>(buf, gx, gy, lane);
+    x_inv_pair<T, NX>(buf, tw, a.nd >= 3 ? &a.gz : nullptr, &a.lap, line, a.Kx, a.kx, lane);
+    x_get<T, NX, PTS>(buf, gz, lap, lane);
+  }
+  bool bad = false;
+#pragma unroll
+  for (int p = 0; p < PTS; ++p) {
+    const bool live = (lane + 64 * p) < NX;
+    double n = a.n0 + dn[p];
+    if (live && !(n > 0.0)) bad = true;
+    double inv = 1.0 / (a.mass * n);
+    u0[p] = m0[p] * inv; u1[p] = m1[p] * inv; u2[p] = m2[p] * inv;
+    // mu_loc with its k = 0 constants (ln n0 / beta, alpha0 / beta) dropped:
+    // only grad(mu) enters the momentum equation (model.f90:99).
+    double v = 0.0;
+    if (a.ideal) v += log1p(dn[p] / a.n0) / a.beta;  // fe_ideal_classical.f90:40
+    if (a.tf || a.grad) {
+      double al = live && n > 0.0 ? fd::alpha_from_ip12(a.fdtab, n / a.cn) : a.alpha0;
+      if (a.tf) v += (al - a.alpha0) / a.beta;       // fe_kinetic_tfk.f90:80-81
+      if (a.grad) {                                  // fe_kinetic_tfk.f90:130-137
+        double xi = fd::xi(a.fdtab, al);
+        double dxidn = fd::dxi(a.fdtab, al) / (0.5 * a.cn * fd::im12(a.fdtab, al));
+        double g2 = gx[p] * gx[p] + gy[p] * gy[p] + gz[p] * gz[p];
+        v -= a.cg * (dxidn * g2 + 2.0 * xi * lap[p]);
+      }
+    }
+    mu[p] = v;
+  }
+  if (bad) a.ctl->nonpos = 1;
+  const double* mm[3] = {m0, m1, m2};
+  const double* uu[3] = {u0, u1, u2};
+  for (int o = 0; o < a.nout; o += 2) {
+    double va[PTS], vb[PTS];
+    for (int q = 0; q < 2; ++q) {
+      if (o + q >= a.nout) break;
+      double* dst = q == 0 ? va : vb;
+      int c = a.code[o + q];
+#pragma unroll
+      for (int p = 0; p < PTS; ++p) {
+        double v;
+        switch (c) {
+          case OA_MU: v = mu[p]; break;
+          case OA_U1: v = u0[p]; break;
+          case OA_U2: v = u1[p]; break;
+          case OA_U3: v = u2[p]; break;
+          case OA_F11: v = mm[0][p] * uu[0][p]; break;
+          case OA_F12: v = mm[0][p] * uu[1][p]; break;
+          case OA_F13: v = mm[0][p] * uu[2][p]; break;
+          case OA_F22: v = mm[1][p] * uu[1][p]; break;
+          case OA_F23: v = mm[1][p] * uu[2][p]; break;
+          default: v = mm[2][p] * uu[2][p]; break;
+        }
+        dst[p] = v;
+      }
+    }
+    const bool two = o + 1 < a.nout;
+    x_fwd_pair<T, NX, PTS>(buf, tw, va, two ? vb : nullptr, a.dst[o], two ? a.dst[o + 1] : nullptr,
+                           line, a.Kx, lane);
+  }
+}
+
+template <class T, int NX>
+__global__ __launch_bounds__(64) void k_phaseB(PhysArgs a, const cpx<T>* __restrict__ twg) {
+  constexpr int PTS = NX >= 64 ? NX / 64 : 1;
+  __shared__ cpx<T> buf[NX];
+  __shared__ cpx<T> tw[NX];
+  const int lane = threadIdx.x;
+  const size_t line = blockIdx.x;
+  const int iy = (int)(line % a.ny), iz = (int)(line / a.ny);
+  for (int j = lane; j < NX; j += 64) tw[j] = twg[j];
+  double dn[PTS], gmx[PTS], gmy[PTS], gmz[PTS];
+  x_inv_pair<T, NX>(buf, tw, &a.n, &a.mux, line, a.Kx, a.kx, lane);
+  x_get<T, NX, PTS>(buf, dn, gmx, lane);
+  if (a.nd >= 2) {
+    x_inv_pair<T, NX>(buf, tw, &a.muy, a.nd >= 3 ? &a.muz : nullptr, line, a.Kx, a.kx, lane);
+    x_get<T, NX, PTS>(buf, gmy, gmz, lane);
+  } else {
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) { gmy[p] = 0.0; gmz[p] = 0.0; }
+  }
+  // external force (external_potential.f90:65-93), at the stage-start time
+  double f[3][PTS];
+  double sf[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int p = 0; p < PTS; ++p) { f[0][p] = 0.0; f[1][p] = 0.0; f[2][p] = 0.0; }
+  if (a.vext_kind == HY_VEXT_MODE || a.vext_kind == HY_VEXT_DRIVEN) {
+    double t = a.vext_td ? a.ctl->ts : 0.0;
+    double amp = a.amp;
+    if (a.vext_td && a.ramp > 0.0) amp = amp * fmin(1.0, t / a.ramp);
+    double yz = a.kvec[1] * (iy * a.dy) + a.kvec[2] * (iz * a.dz);
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) {
+      int x = lane + 64 * p;
+      double ph = a.kvec[0] * (x * a.dx) + yz - a.omega * t;
+      double s = amp * sin(ph);
+      f[0][p] = s * a.kvec[0]; f[1][p] = s * a.kvec[1]; f[2][p] = s * a.kvec[2];
+    }
+  } else if (a.vext_kind == HY_VEXT_GAUSSIAN) {
+    const size_t nr = (size_t)NX * a.ny * a.nz;
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) {
+      int x = lane + 64 * p;
+      if (x < NX)
+        for (int d = 0; d < 3; ++d) f[d][p] = a.fext[d * nr + line * NX + x];
+    }
+  }
+  double h[3][PTS];
+#pragma unroll
+  for (int p = 0; p < PTS; ++p) {
+    const bool live = (lane + 64 * p) < NX;
+    double n = a.n0 + dn[p];
+    h[0][p] = n * (f[0][p] - gmx[p]);
+    h[1][p] = n * (f[1][p] - gmy[p]);
+    h[2][p] = n * (f[2][p] - gmz[p]);
+    if (live) { sf[0] += n * f[0][p]; sf[1] += n * f[1][p]; sf[2] += n * f[2][p]; }
+  }
+  if (a.vext_kind != HY_VEXT_NONE) {
+    for (int d = 0; d < 3; ++d) {
+      double s = wave_sum(sf[d]);
+      if (lane == 0) a.red[line * 3 + d] = s;
+    }
+  }
+  // Maxwell stress advection dPi_c = -sum_d u_d d_d Pi_c (model.f90:126-134)
+  double adv[6][PTS];
+  if (a.evolve) {
+    double m0[PTS], m1[PTS], m2[PTS], u[3][PTS];
+    x_inv_pair<T, NX>(buf, tw, &a.m[0], a.nd >= 2 ? &a.m[1] : nullptr, line, a.Kx, a.kx, lane);
+    x_get<T, NX, PTS>(buf, m0, m1, lane);
+    if (a.nd >= 3) {
+      x_inv_pair<T, NX>(buf, tw, &a.m[2], nullptr, line, a.Kx, a.kx, lane);
+      x_get<T, NX, PTS>(buf, m2, nullptr, lane);
+    } else {
+#pragma unroll
+      for (int p = 0; p < PTS; ++p) m2[p] = 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) {
+      double inv = 1.0 / (a.mass * (a.n0 + dn[p]));
+      u[0][p] = m0[p] * inv; u[1][p] = m1[p] * inv; u[2][p] = m2[p] * inv;
+    }
+    for (int c = 0; c < 6; ++c) {
+      double dx_[PTS], dy_[PTS], dz_[PTS];
+      x_inv_pair<T, NX>(buf, tw, &a.px[c], a.nd >= 2 ? &a.py[c] : nullptr, line, a.Kx, a.kx, lane);
+      x_get<T, NX, PTS>(buf, dx_, dy_, lane);
+      if (a.nd >= 3) {
+        x_inv_pair<T, NX>(buf, tw, &a.pz[c], nullptr, line, a.Kx, a.kx, lane);
+        x_get<T, NX, PTS>(buf, dz_, nullptr, lane);
+      }
+#pragma unroll
+      for (int p = 0; p < PTS; ++p) {
+        double v = 0.0;
+        v = v - u[0][p] * dx_[p];
+        if (a.nd >= 2) v = v - u[1][p] * dy_[p];
+        if (a.nd >= 3) v = v - u[2][p] * dz_[p];
+        adv[c][p] = v;
+      }
+    }
+  }
+  for (int o = 0; o < a.nout; o += 2) {
+    double va[PTS], vb[PTS];
+    for (int q = 0; q < 2; ++q) {
+      if (o + q >= a.nout) break;
+      double* dst = q == 0 ? va : vb;
+      int c = a.code[o + q];
+#pragma unroll
+      for (int p = 0; p < PTS; ++p) dst[p] = c < 3 ? h[c][p] : adv[c - 3][p];
+    }
+    const bool two = o + 1 < a.nout;
+    x_fwd_pair<T, NX, PTS>(buf, tw, va, two ? vb : nullptr, a.dst[o], two ? a.dst[o + 1] : nullptr,
+                           line, a.Kx, lane);
+  }
+}
+
+// max |u_d| over a line (model.f90:171-172)
+template <class T, int NX>
+__global__ __launch_bounds__(64) void k_umax(PhysArgs a, const cpx<T>* __restrict__ twg) {
+  constexpr int PTS = NX >= 64 ? NX / 64 : 1;
+  __shared__ cpx<T> buf[NX];
+  __shared__ cpx<T> tw[NX];
+  const int lane = threadIdx.x;
+  const size_t line = blockIdx.x;
+  for (int j = lane; j < NX; j += 64) tw[j] = twg[j];
+  double dn[PTS], m0[PTS], m1[PTS], m2[PTS];
+  x_inv_pair<T, NX>(buf, tw, &a.n, &a.m[0], line, a.Kx, a.kx, lane);
+  x_get<T, NX, PTS>(buf, dn, m0, lane);
+  if (a.nd >= 2) {
+    x_inv_pair<T, NX>(buf, tw, &a.m[1], a.nd >= 3 ? &a.m[2] : nullptr, line, a.Kx, a.kx, lane);
+    x_get<T, NX, PTS>(buf, m1, m2, lane);
+  }
+  double um = 0.0;
+#pragma unroll
+  for (int p = 0; p < PTS; ++p) {
+    if (lane + 64 * p >= NX) continue;
+    double inv = 1.0 / (a.mass * (a.n0 + dn[p]));
+    um = fmax(um, fabs(m0[p] * inv));
+    if (a.nd >= 2) um = fmax(um, fabs(m1[p] * inv));
+    if (a.nd >= 3) um = fmax(um, fabs(m2[p] * inv));
+  }
+  um = wave_max(um);
+  if (lane == 0) a.red[line] = um;
+}
+
+// compute_dt from the per-line maxima (model.f90:169-181)
+__global__ void k_dt(const double* __restrict__ red, int nlines, Ctl* c, double wmax, double numax,
+                     double dxmin, double cfl) {
+  __shared__ double sh[256];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < nlines; i += blockDim.x) m = fmax(m, red[i]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double umax = sh[0];
+    const double huge = 1.7976931348623157e308;
+    double dtm = huge;
+    if (wmax > 0.0) dtm = fmin(dtm, 1.7 / wmax);
+    if (numax > 0.0) dtm = fmin(dtm, 2.5 / numax);
+    if (umax > 0.0) dtm = fmin(dtm, dxmin / umax);
+    c->umax = umax;
+    if (dtm == huge) c->nodt = 1;
+    else c->dt = cfl * dtm;
+  }
+}
+
+// sum over lines of `ncomp` partials, scaled: out[c] = scale * sum_l red[l*ncomp + c]
+__global__ void k_reduce_lines(const double* __restrict__ red, int nlines, int ncomp, double scale,
+                               double* __restrict__ out) {
+  __shared__ double sh[256];
+  for (int c = 0; c < ncomp; ++c) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nlines; i += blockDim.x) s += red[(size_t)i * ncomp + c];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sh[0] * scale;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------ spectral kernels
+struct SpecArgs {
+  int Kx, Ky, Kz, My, Mz, ny, nz, nd;
+  const double *kx, *ky, *kz;
+  size_t nt;   // Kx*Ky*Kz
+};
+__device__ inline void spec_k(const SpecArgs& a, size_t e, double k[3]) {
+  int ix = (int)(e % a.Kx);
+  int ty = (int)((e / a.Kx) % a.Ky);
+  int tz = (int)(e / ((size_t)a.Kx * a.Ky));
+  int jy = ty <= a.My ? ty : ty - a.Ky + a.ny;
+  int jz = tz <= a.Mz ? tz : tz - a.Kz + a.nz;
+  k[0] = a.kx[ix];
+  k[1] = a.ky[jy];
+  k[2] = a.kz[jz];
+}
+template <class T>
+__device__ inline c64 ld(const void* p, size_t i) {
+  return cvt<double>(reinterpret_cast<const cpx<T>*>(p)[i]);
+}
+
+// Newtonian stress Pi0 from u_hat (stress.f90:76-82) evaluated spectrally
+__device__ inline void pi0_hat(const c64* u, const double* k, double eta, double xi, c64* out) {
+  auto g = [&](int i, int j) { return mk<double>(-k[j] * u[i].y, k[j] * u[i].x); };
+  c64 divu = g(0, 0) + g(1, 1) + g(2, 2);
+  for (int c = 0; c < 3; ++c)
+    out[c] = scal(scal(g(c, c), 2.0) - scal(divu, 2.0 / 3.0), eta) + scal(divu, xi);
+  out[3] = scal(g(0, 1) + g(1, 0), eta);
+  out[4] = scal(g(0, 2) + g(2, 0), eta);
+  out[5] = scal(g(1, 2) + g(2, 1), eta);
+}
+
+struct K1Args {
+  SpecArgs s;
+  const void* fmu;        // F(mu_loc) (compute type)
+  const c64* N;           // state n_hat
+  const double* kcomb;    // hartree v(k) - ry c_cor(k)/beta on the kept box
+  double n0;
+  void* muhat;            // out (compute type)
+  // Maxwell
+  int evolve, relax;
+  c64* P;                 // state Pi_hat (6)
+  const void* fu[3];      // F(u) of this stage (compute type)
+  c64* uold;              // u_hat at the previous stage start (3, FP64)
+  double eta, xi;
+  const Ctl* ctl;
+};
+
+template <class T>
+__global__ void k_spec1(K1Args a) {
+  const size_t nt = a.s.nt;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nt; e += (size_t)gridDim.x * blockDim.x) {
+    c64 nh = a.N[e];
+    if (e == 0) nh.x -= a.n0;  // fe_correlation_ry.f90:78
+    c64 mu = ld<T>(a.fmu, e) + scal(nh, a.kcomb[e]);
+    if (e == 0) mu = mk<double>(0, 0);  // only grad(mu) is used
+    reinterpret_cast<cpx<T>*>(a.muhat)[e] = cvt<T>(mu);
+    if (a.evolve) {
+      double k[3];
+      spec_k(a.s, e, k);
+      c64 un[3];
+      for (int d = 0; d < 3; ++d) un[d] = d < a.s.nd ? ld<T>(a.fu[d], e) : mk<double>(0, 0);
+      if (a.relax) {  // model.f90:209-214, applied with this stage's u = u after the update
+        c64 uo[3] = {a.uold[e], a.uold[nt + e], a.uold[2 * nt + e]};
+        c64 p0n[6], p0o[6];
+        pi0_hat(un, k, a.eta, a.xi, p0n);
+        pi0_hat(uo, k, a.eta, a.xi, p0o);
+        const double fac = a.ctl->relax_fac, phi = a.ctl->relax_phi;
+        for (int c = 0; c < 6; ++c) {
+          c64 pc = a.P[c * nt + e];
+          a.P[c * nt + e] = p0n[c] - scal(p0n[c] - p0o[c], phi) + scal(pc - p0o[c], fac);
+        }
+      }
+      for (int d = 0; d < 3; ++d) a.uold[d * nt + e] = un[d];
+    }
+  }
+}
+
+struct K2Args {
+  SpecArgs s;
+  c64* S;                 // state: n, mom(3), Pi(6) (FP64)
+  c64* R;                 // R_old (same layout), or the output in rhs mode
+  const void* fl[6];      // F(flux_ij): 11 12 13 22 23 33
+  const void* fh[3];      // F(H_i)
+  const void* fu[3];      // F(u) (Newtonian stress)
+  const void* fadv[6];    // F(adv_c) (Maxwell)
+  double mass, eta, xi;
+  int stress_newt, evolve, ext_active;
+  int stage;              // 0..2
+  int rhs_only;
+  const Ctl* ctl;
+};
+
+template <class T>
+__global__ void k_spec2(K2Args a) {
+  const size_t nt = a.s.nt;
+  const int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  const int idx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};  // stress.f90:92
+  const int nd = a.s.nd;
+  const double dt = a.ctl->dt;
+  const double ra = c_rk_a[a.stage], rb = c_rk_b[a.stage];
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nt; e += (size_t)gridDim.x * blockDim.x) {
+    double k[3];
+    spec_k(a.s, e, k);
+    c64 M[3] = {a.S[nt + e], a.S[2 * nt + e], a.S[3 * nt + e]};
+    c64 r[10];
+    // continuity: dn = -div(mom)/m (model.f90:95-96)
+    c64 km = scal(M[0], k[0]) + scal(M[1], k[1]) + scal(M[2], k[2]);
+    r[0] = mk<double>(km.y / a.mass, -km.x / a.mass);
+    // stress divergence f_i = d_j Pi_ij
+    c64 pih[6];
+    bool have_pi = false;
+    if (a.evolve) {
+      for (int c = 0; c < 6; ++c) pih[c] = a.S[(4 + c) * nt + e];
+      have_pi = true;
+    } else if (a.stress_newt) {
+      c64 u[3];
+      for (int d = 0; d < 3; ++d) u[d] = d < nd ? ld<T>(a.fu[d], e) : mk<double>(0, 0);
+      pi0_hat(u, k, a.eta, a.xi, pih);
+      have_pi = true;
+    }
+    for (int i = 0; i < 3; ++i) {
+      c64 acc = mk<double>(0, 0);
+      if (i < nd) {
+        acc = ld<T>(a.fh[i], e);  // F(n (f_ext - d_i mu))
+        for (int j = 0; j < nd; ++j) {  // - i k_j F(m_i u_j)
+          c64 fv = ld<T>(a.fl[sym[i][j]], e);
+          acc.x += k[j] * fv.y;
+          acc.y -= k[j] * fv.x;
+        }
+        if (have_pi) {
+          for (int j = 0; j < nd; ++j) {
+            c64 pv = pih[idx[i][j]];
+            acc.x -= k[j] * pv.y;
+            acc.y += k[j] * pv.x;
+          }
+        }
+        // k = 0: only the external force survives (the internal force has its mean
+        // removed, model.f90:119-122)
+        if (e == 0) acc = mk<double>(a.ext_active ? a.ctl->nf[i] : 0.0, 0.0);
+      }
+      r[1 + i] = acc;
+    }
+    int nf = 4;
+    if (a.evolve) {
+      for (int c = 0; c < 6; ++c) r[4 + c] = ld<T>(a.fadv[c], e);
+      nf = 10;
+    }
+    if (a.rhs_only) {
+      for (int f = 0; f < nf; ++f) a.R[f * nt + e] = r[f];
+      continue;
+    }
+    for (int f = 0; f < nf; ++f) {
+      c64 s = a.S[f * nt + e];
+      if (a.stage == 0) {
+        s = s + scal(r[f], dt * ra);
+      } else {
+        c64 ro = a.R[f * nt + e];
+        s = s + scal(scal(r[f], ra) + scal(ro, rb), dt);
+      }
+      a.S[f * nt + e] = s;
+      a.R[f * nt + e] = r[f];
+    }
+  }
+}
+
+// probe: spec_mode of n/n0 and mom_x (diagnostics.f90:70-77) after a step
+__global__ void k_probe(const Ctl* c, const c64* S, size_t nt, long long idx, double n0, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  c64 a = idx >= 0 ? S[idx] : mk<double>(0, 0);
+  c64 b = idx >= 0 ? S[nt + idx] : mk<double>(0, 0);
+  out[0] = c->t;
+  out[1] = a.x / n0; out[2] = a.y / n0;
+  out[3] = b.x; out[4] = b.y;
+  out[5] = c->dt;
+}
+
+// full half spectrum <-> kept box
+__global__ void k_full_to_trunc(const c64* __restrict__ full, c64* __restrict__ tr, SpecArgs a, int nkx) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < a.nt; e += (size_t)gridDim.x * blockDim.x) {
+    int ix = (int)(e % a.Kx), ty = (int)((e / a.Kx) % a.Ky), tz = (int)(e / ((size_t)a.Kx * a.Ky));
+    int jy = ty <= a.My ? ty : ty - a.Ky + a.ny;
+    int jz = tz <= a.Mz ? tz : tz - a.Kz + a.nz;
+    tr[e] = full[((size_t)jz * a.ny + jy) * nkx + ix];
+  }
+}
+__global__ void k_trunc_to_full(const c64* __restrict__ tr, c64* __restrict__ full, SpecArgs a, int nkx) {
+  size_t n = (size_t)nkx * a.ny * a.nz;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    int i = (int)(e % nkx), j = (int)((e / nkx) % a.ny), k = (int)(e / ((size_t)nkx * a.ny));
+    int my = j <= a.ny / 2 ? j : j - a.ny, mz = k <= a.nz / 2 ? k : k - a.nz;
+    c64 v = mk<double>(0, 0);
+    if (i < a.Kx && abs(my) <= a.My && abs(mz) <= a.Mz) {
+      int ty = my >= 0 ? my : my + a.Ky, tz = mz >= 0 ? mz : mz + a.Kz;
+      v = tr[((size_t)tz * a.Ky + ty) * a.Kx + i];
+    }
+    full[e] = v;
+  }
+}
+
+// ------------------------------------------------------------ the model
+struct Model {
+  Grid g;
+  int prec = HY_PREC_FP64;
+  double mass = 1, n0 = 0;
+  int ideal = 0, tf = 0, grad = 0, hartree = 0, ry = 0;
+  double beta = 3, cn = 0, alpha0 = 0, cg = 0, e2 = 1.0 / 3.0, kappa = 0;
+  std::vector<hy_term_desc> terms;
+  std::vector<std::vector<double>> ktabs, ctabs;
+  int closure = 0, evolve = 0, stress_newt = 0;
+  double eta = 0, xi = 0, tau = 0, eta_l = 0;
+  int vext_kind = 0, vext_td = 0;
+  double vamp = 0, vomega = 0, vwidth = 1, vramp = 0, kvec[3] = {0, 0, 0};
+  int vmode[3] = {1, 0, 0};
+  double dt_fixed = 0, cfl = 0.4;
+  double wmax = 0, numax = 0, dxmin = 0;
+  // device
+  c64* S = nullptr;     // 10 * nt
+  c64* R = nullptr;     // 10 * nt
+  c64* uold = nullptr;  // 3 * nt
+  double* kcomb = nullptr;
+  double* fext = nullptr;
+  double* fdtab = nullptr;
+  void* xb = nullptr;   // x-line slots (compute type), each Kx*ny*nz
+  void* zb = nullptr;   // z-pass outputs, each Kx*Ky*nz
+  void* fb = nullptr;   // truncated spectra, each nt
+  double* red = nullptr;
+  Ctl* ctl = nullptr;
+  double* probe = nullptr;
+  int probe_cap = 0;
+  int nxslots = 0, nzslots = 0, nfslots = 0;
+  size_t xsz = 0, zsz = 0, nt = 0;
+  size_t esz = 16;      // compute element size
+  hipStream_t st = nullptr;
+  int relax_pending = 0;
+  int irecompute = 0;
+  // host mirror of the time
+  double h_t = 0, h_dt = 0;
+  int h_step = 0;
+};
+
+static bool mixed(const Model& m) { return m.prec == HY_PREC_MIXED; }
+static void* xslot(Model& m, int i) { return (char*)m.xb + m.xsz * m.esz * i; }
+static void* zslot(Model& m, int i) { return (char*)m.zb + m.zsz * m.esz * i; }
+static void* fslot(Model& m, int i) { return (char*)m.fb + m.nt * m.esz * i; }
+static c64* sfield(Model& m, int f) { return m.S + m.nt * f; }
+
+static SpecArgs spec_args(const Model& m) {
+  SpecArgs a;
+  const Grid& g = m.g;
+  a.Kx = g.Kx; a.Ky = g.Ky; a.Kz = g.Kz; a.My = g.My; a.Mz = g.Mz; a.ny = g.ny; a.nz = g.nz;
+  a.nd = g.ndim; a.kx = g.d_kx; a.ky = g.d_ky; a.kz = g.d_kz; a.nt = m.nt;
+  return a;
+}
+static int nblocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 16384); }
+
+// x-line slots
+enum XS { XS_N = 0, XS_M1, XS_M2, XS_M3, XS_GY, XS_GZ, XS_LAP, XS_A0 = 7 /*10 outputs*/ };
+// phase B inputs reuse XS_A0.., then outputs
+static int XS_B_MU() { return 7; }
+static int XS_B_MUY() { return 8; }
+static int XS_B_MUZ() { return 9; }
+static int XS_B_P(int c) { return 10 + c; }
+static int XS_B_PY(int c) { return 16 + c; }
+static int XS_B_PZ(int c) { return 22 + c; }
+static int XS_B_OUT(const Model& m) { return m.evolve ? 28 : 10; }
+// truncated-spectrum slots
+static int FS_A(int o) { return o; }          // phase A outputs, 0..9
+static int FS_B(int o) { return 10 + o; }     // phase B outputs, 10..18
+static const int FS_MU = 19;
+static const int FS_RHS = 20;                 // rhs-only output: 10 slots of c64 (20..)
+
+// ---- inverse chains: truncated spectrum -> x-line slot (returns the XIn)
+struct InvSpec {
+  const void* src;
+  int src64;
+  double k0off;
+  int mx, my, mz;  // XMul / MulKind per axis
+  int xslot;       // destination x-line slot (3-D/2-D)
+};
+
+static int inv_chain(Model& m, const std::vector<InvSpec>& fs, std::vector<XIn>& xin) {
+  const Grid& g = m.g;
+  xin.clear();
+  const int nd = g.ndim;
+  if (g.nz > 1) {
+    std::vector<PencilField> zf, yf;
+    for (size_t i = 0; i < fs.size(); ++i) {
+      const auto& f = fs[i];
+      zf.push_back({f.src, zslot(m, (int)i), f.mz, f.src64, f.k0off});
+      yf.push_back({zslot(m, (int)i), xslot(m, f.xslot), f.my, 0, 0.0});
+    }
+    if ((int)fs.size() > m.nzslots) return set_error(HY_ERR_INVALID, "internal: z slots");
+    HY_TRY(pencil_pass(g, 2, true, true, g.Kx * g.Ky, 1, zf.data(), (int)zf.size(), m.st));
+    if (g.ny > 1) HY_TRY(pencil_pass(g, 1, true, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st));
+    for (auto& f : fs) xin.push_back({xslot(m, f.xslot), 0, f.mx, 0.0});
+    // ny == 1 with nz > 1: the z output is already (Kx, 1, nz) x-lines
+    if (g.ny == 1)
+      for (size_t i = 0; i < fs.size(); ++i) xin[i].p = zslot(m, (int)i);
+  } else if (g.ny > 1) {
+    std::vector<PencilField> yf;
+    for (auto& f : fs) yf.push_back({f.src, xslot(m, f.xslot), f.my, f.src64, f.k0off});
+    HY_TRY(pencil_pass(g, 1, true, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st));
+    for (auto& f : fs) xin.push_back({xslot(m, f.xslot), 0, f.mx, 0.0});
+  } else {
+    for (auto& f : fs) xin.push_back({f.src, f.src64, f.mx, f.k0off});
+  }
+  (void)nd;
+  return HY_OK;
+}
+
+// ---- forward chains: x-line slot -> truncated spectrum (returns the x kernel destinations)
+static void* fwd_dst(Model& m, int xs, int fs) {
+  const Grid& g = m.g;
+  return (g.ny > 1 || g.nz > 1) ? xslot(m, xs) : fslot(m, fs);
+}
+static int fwd_chain(Model& m, const std::vector<std::pair<int, int>>& f /* (xslot, fslot) */) {
+  const Grid& g = m.g;
+  if (f.empty()) return HY_OK;
+  if (g.nz > 1) {
+    std::vector<PencilField> yf, zf;
+    for (size_t i = 0; i < f.size(); ++i) {
+      if (g.ny > 1) yf.push_back({xslot(m, f[i].first), zslot(m, (int)i), MUL_NONE, 0, 0.0});
+      const void* zin = g.ny > 1 ? zslot(m, (int)i) : xslot(m, f[i].first);
+      zf.push_back({zin, fslot(m, f[i].second), MUL_NONE, 0, 0.0});
+    }
+    if ((int)f.size() > m.nzslots) return set_error(HY_ERR_INVALID, "internal: z slots");
+    if (g.ny > 1) HY_TRY(pencil_pass(g, 1, false, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st));
+    HY_TRY(pencil_pass(g, 2, false, true, g.Kx * g.Ky, 1, zf.data(), (int)zf.size(), m.st));
+  } else if (g.ny > 1) {
+    std::vector<PencilField> yf;
+    for (auto& p : f) yf.push_back({xslot(m, p.first), fslot(m, p.second), MUL_NONE, 0, 0.0});
+    HY_TRY(pencil_pass(g, 1, false, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st));
+  }
+  return HY_OK;
+}
+
+// ---- x kernel dispatch
+enum XK { XK_A = 0, XK_B = 1, XK_UMAX = 2 };
+template <class T, int NX>
+static void launch_xk(int which, const PhysArgs& a, int nlines, const cpx<T>* tw, hipStream_t s) {
+  if (which == XK_A) hipLaunchKernelGGL((k_phaseA<T, NX>), dim3(nlines), dim3(64), 0, s, a, tw);
+  else if (which == XK_B) hipLaunchKernelGGL((k_phaseB<T, NX>), dim3(nlines), dim3(64), 0, s, a, tw);
+  else hipLaunchKernelGGL((k_umax<T, NX>), dim3(nlines), dim3(64), 0, s, a, tw);
+}
+template <class T>
+static int dispatch_xk(int which, const Model& m, const PhysArgs& a, const cpx<T>* tw) {
+  const int nlines = m.g.ny * m.g.nz;
+  switch (m.g.nx) {
+#define HY_XK(LL) case LL: launch_xk<T, LL>(which, a, nlines, tw, m.st); break;
+    HY_XK(4) HY_XK(8) HY_XK(12) HY_XK(16) HY_XK(24) HY_XK(32) HY_XK(48) HY_XK(64) HY_XK(96)
+    HY_XK(128) HY_XK(256)
+#undef HY_XK
+    default: return set_error(HY_ERR_UNSUPPORTED, "x kernel: unsupported nx");
+  }
+  HY_HIP(hipGetLastError());
+  return HY_OK;
+}
+static int run_xk(Model& m, int which, const PhysArgs& a) {
+  const TwiddleSet& t = m.g.tw(m.g.nx);
+  if (mixed(m)) return dispatch_xk<float>(which, m, a, t.d32);
+  return dispatch_xk<double>(which, m, a, t.d64);
+}
+
+static PhysArgs phys_base(Model& m) {
+  PhysArgs a;
+  std::memset(&a, 0, sizeof(a));
+  const Grid& g = m.g;
+  a.nd = g.ndim; a.Kx = g.Kx; a.ny = g.ny; a.nz = g.nz;
+  a.dx = g.dx; a.dy = g.dy; a.dz = g.dz; a.kx = g.d_kx;
+  a.n0 = m.n0; a.mass = m.mass; a.beta = m.beta; a.cn = m.cn; a.alpha0 = m.alpha0; a.cg = m.cg;
+  a.ideal = m.ideal; a.tf = m.tf; a.grad = m.grad;
+  a.fdtab = m.fdtab; a.ctl = m.ctl; a.evolve = m.evolve;
+  a.vext_kind = m.vext_kind; a.vext_td = m.vext_td; a.amp = m.vamp; a.ramp = m.vramp;
+  a.omega = m.vomega; a.kvec[0] = m.kvec[0]; a.kvec[1] = m.kvec[1]; a.kvec[2] = m.kvec[2];
+  a.fext = m.fext; a.red = m.red;
+  return a;
+}
+
+// state (n, mom) -> x-lines of n - n0 and mom, plus the electron gradient inputs
+static int stage_inputs(Model& m, PhysArgs& a, bool with_grad) {
+  const int nd = m.g.ndim;
+  std::vector<InvSpec> fs;
+  fs.push_back({sfield(m, 0), 1, -m.n0, XM_NONE, MUL_NONE, MUL_NONE, XS_N});
+  for (int d = 0; d < nd; ++d) fs.push_back({sfield(m, 1 + d), 1, 0.0, XM_NONE, MUL_NONE, MUL_NONE, XS_M1 + d});
+  const int nbase = (int)fs.size();
+  if (with_grad) {
+    if (nd >= 2) fs.push_back({sfield(m, 0), 1, 0.0, XM_NONE, MUL_IK, MUL_NONE, XS_GY});
+    if (nd >= 3) fs.push_back({sfield(m, 0), 1, 0.0, XM_NONE, MUL_NONE, MUL_IK, XS_GZ});
+    // Laplacian: -k^2 at the first pass of the chain
+    if (nd == 1) fs.push_back({sfield(m, 0), 1, 0.0, XM_NEGK2, MUL_NONE, MUL_NONE, XS_LAP});
+    else if (nd == 2) fs.push_back({sfield(m, 0), 1, 0.0, XM_NONE, MUL_NEGK2, MUL_NONE, XS_LAP});
+    else fs.push_back({sfield(m, 0), 1, 0.0, XM_NONE, MUL_NONE, MUL_NEGK2, XS_LAP});
+  }
+  std::vector<XIn> xin;
+  HY_TRY(inv_chain(m, fs, xin));
+  a.n = xin[0];
+  for (int d = 0; d < nd; ++d) a.m[d] = xin[1 + d];
+  if (with_grad) {
+    int q = nbase;
+    a.gx = xin[0];
+    a.gx.k0off = 0.0;
+    a.gx.mul = XM_IK;
+    if (nd >= 2) a.gy = xin[q++];
+    if (nd >= 3) a.gz = xin[q++];
+    a.lap = xin[q++];
+  }
+  return HY_OK;
+}
+
+static int launch_k1(Model& m, int relax);
+static int launch_k2(Model& m, int stage, int rhs_only);
+
+// One RHS evaluation up to K2; the stage time is ctl->ts.
+static int stage(Model& m, int s, int rhs_only) {
+  const Grid& g = m.g;
+  const int nd = g.ndim;
+  // ---- phase A
+  PhysArgs a = phys_base(m);
+  HY_TRY(stage_inputs(m, a, m.grad));
+  const bool need_u = m.stress_newt || m.evolve;
+  std::vector<std::pair<int, int>> fwd;
+  auto addA = [&](int code) {
+    int o = a.nout++;
+    a.code[o] = code;
+    a.dst[o] = fwd_dst(m, XS_A0 + o, FS_A(code));
+    fwd.push_back({XS_A0 + o, FS_A(code)});
+  };
+  addA(OA_MU);
+  if (need_u) for (int d = 0; d < nd; ++d) addA(OA_U1 + d);
+  const int fcode[3][3] = {{OA_F11, OA_F12, OA_F13}, {OA_F12, OA_F22, OA_F23}, {OA_F13, OA_F23, OA_F33}};
+  for (int i = 0; i < nd; ++i)
+    for (int j = i; j < nd; ++j) addA(fcode[i][j]);
+  HY_TRY(run_xk(m, XK_A, a));
+  HY_TRY(fwd_chain(m, fwd));
+  // ---- K1: mu_hat (+ pending Maxwell relaxation)
+  HY_TRY(launch_k1(m, m.relax_pending && !rhs_only));
+  if (m.evolve && !rhs_only) m.relax_pending = 0;
+  // ---- phase B
+  PhysArgs b = phys_base(m);
+  b.n = a.n;
+  for (int d = 0; d < nd; ++d) b.m[d] = a.m[d];
+  std::vector<InvSpec> fs;
+  fs.push_back({fslot(m, FS_MU), 0, 0.0, XM_IK, MUL_NONE, MUL_NONE, XS_B_MU()});
+  if (nd >= 2) fs.push_back({fslot(m, FS_MU), 0, 0.0, XM_NONE, MUL_IK, MUL_NONE, XS_B_MUY()});
+  if (nd >= 3) fs.push_back({fslot(m, FS_MU), 0, 0.0, XM_NONE, MUL_NONE, MUL_IK, XS_B_MUZ()});
+  if (m.evolve) {
+    for (int c = 0; c < 6; ++c) {
+      fs.push_back({sfield(m, 4 + c), 1, 0.0, XM_IK, MUL_NONE, MUL_NONE, XS_B_P(c)});
+      if (nd >= 2) fs.push_back({sfield(m, 4 + c), 1, 0.0, XM_NONE, MUL_IK, MUL_NONE, XS_B_PY(c)});
+      if (nd >= 3) fs.push_back({sfield(m, 4 + c), 1, 0.0, XM_NONE, MUL_NONE, MUL_IK, XS_B_PZ(c)});
+    }
+  }
+  std::vector<XIn> xin;
+  HY_TRY(inv_chain(m, fs, xin));
+  int q = 0;
+  b.mux = xin[q++];
+  if (nd >= 2) b.muy = xin[q++];
+  if (nd >= 3) b.muz = xin[q++];
+  if (m.evolve)
+    for (int c = 0; c < 6; ++c) {
+      b.px[c] = xin[q++];
+      if (nd >= 2) b.py[c] = xin[q++];
+      if (nd >= 3) b.pz[c] = xin[q++];
+    }
+  std::vector<std::pair<int, int>> fwdb;
+  const int xo = XS_B_OUT(m);
+  auto addB = [&](int code) {
+    int o = b.nout++;
+    b.code[o] = code;
+    b.dst[o] = fwd_dst(m, xo + o, FS_B(code));
+    fwdb.push_back({xo + o, FS_B(code)});
+  };
+  for (int i = 0; i < nd; ++i) addB(OB_H1 + i);
+  if (m.evolve) for (int c = 0; c < 6; ++c) addB(OB_ADV1 + c);
+  HY_TRY(run_xk(m, XK_B, b));
+  HY_TRY(fwd_chain(m, fwdb));
+  if (m.vext_kind != HY_VEXT_NONE) {
+    hipLaunchKernelGGL(k_reduce_lines, dim3(1), dim3(256), 0, m.st, m.red, g.ny * g.nz, 3,
+                       1.0 / (double)g.nreal(), m.ctl->nf);
+    HY_HIP(hipGetLastError());
+  }
+  // ---- K2: RHS assembly and the RK update
+  HY_TRY(launch_k2(m, s, rhs_only));
+  return HY_OK;
+}
+
+static int launch_k1(Model& m, int relax) {
+  K1Args a;
+  std::memset(&a, 0, sizeof(a));
+  a.s = spec_args(m);
+  a.fmu = fslot(m, FS_A(OA_MU));
+  a.N = sfield(m, 0);
+  a.kcomb = m.kcomb;
+  a.n0 = m.n0;
+  a.muhat = fslot(m, FS_MU);
+  a.evolve = m.evolve;
+  a.relax = relax;
+  a.P = sfield(m, 4);
+  for (int d = 0; d < 3; ++d) a.fu[d] = fslot(m, FS_A(OA_U1 + d));
+  a.uold = m.uold;
+  a.eta = m.eta; a.xi = m.xi;
+  a.ctl = m.ctl;
+  if (mixed(m)) hipLaunchKernelGGL(k_spec1<float>, dim3(nblocks(m.nt)), dim3(256), 0, m.st, a);
+  else hipLaunchKernelGGL(k_spec1<double>, dim3(nblocks(m.nt)), dim3(256), 0, m.st, a);
+  HY_HIP(hipGetLastError());
+  return HY_OK;
+}
+
+static int launch_k2(Model& m, int stage_idx, int rhs_only) {
+  K2Args a;
+  std::memset(&a, 0, sizeof(a));
+  a.s = spec_args(m);
+  a.S = m.S;
+  a.R = rhs_only ? reinterpret_cast<c64*>(fslot(m, FS_RHS)) : m.R;
+  const int fo[6] = {OA_F11, OA_F12, OA_F13, OA_F22, OA_F23, OA_F33};
+  for (int i = 0; i < 6; ++i) a.fl[i] = fslot(m, FS_A(fo[i]));
+  for (int i = 0; i < 3; ++i) a.fh[i] = fslot(m, FS_B(OB_H1 + i));
+  for (int i = 0; i < 3; ++i) a.fu[i] = fslot(m, FS_A(OA_U1 + i));
+  for (int c = 0; c < 6; ++c) a.fadv[c] = fslot(m, FS_B(OB_ADV1 + c));
+  a.mass = m.mass; a.eta = m.eta; a.xi = m.xi;
+  a.stress_newt = m.stress_newt; a.evolve = m.evolve;
+  a.ext_active = m.vext_kind != HY_VEXT_NONE;
+  a.stage = stage_idx;
+  a.rhs_only = rhs_only;
+  a.ctl = m.ctl;
+  if (mixed(m)) hipLaunchKernelGGL(k_spec2<float>, dim3(nblocks(m.nt)), dim3(256), 0, m.st, a);
+  else hipLaunchKernelGGL(k_spec2<double>, dim3(nblocks(m.nt)), dim3(256), 0, m.st, a);
+  HY_HIP(hipGetLastError());
+  return HY_OK;
+}
+
+// Pending Maxwell relaxation after the last stage: needs u_hat of the new state.
+static int finish_relax(Model& m) {
+  if (!m.evolve || !m.relax_pending) return HY_OK;
+  const int nd = m.g.ndim;
+  PhysArgs a = phys_base(m);
+  HY_TRY(stage_inputs(m, a, false));
+  std::vector<std::pair<int, int>> fwd;
+  a.nout = 0;
+  for (int d = 0; d < nd; ++d) {
+    int o = a.nout++;
+    a.code[o] = OA_U1 + d;
+    a.dst[o] = fwd_dst(m, XS_A0 + o, FS_A(OA_U1 + d));
+    fwd.push_back({XS_A0 + o, FS_A(OA_U1 + d)});
+  }
+  HY_TRY(run_xk(m, XK_A, a));
+  HY_TRY(fwd_chain(m, fwd));
+  // mu is not recomputed here: K1 rewrites mu_hat from the stale F(mu_loc) but
+  // it is never read before the next stage recomputes it.
+  HY_TRY(launch_k1(m, 1));
+  m.relax_pending = 0;
+  return HY_OK;
+}
+
+__global__ void k_set_dt(Ctl* c, double dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) c->dt = dt;
+}
+
+static int compute_dt_dev(Model& m) {
+  if (m.dt_fixed > 0.0) {  // model.f90:155-158
+    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, m.st, m.ctl, m.dt_fixed);
+    HY_HIP(hipGetLastError());
+    return HY_OK;
+  }
+  PhysArgs a = phys_base(m);
+  HY_TRY(stage_inputs(m, a, false));
+  HY_TRY(run_xk(m, XK_UMAX, a));
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(256), 0, m.st, m.red, m.g.ny * m.g.nz, m.ctl, m.wmax,
+                     m.numax, m.dxmin, m.cfl);
+  HY_HIP(hipGetLastError());
+  return HY_OK;
+}
+
+static int step_once(Model& m, double tend) {
+  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, m.st, m.ctl, (int)CTL_STEP_BEGIN, 0, m.tau, m.evolve, tend);
+  for (int s = 0; s < 3; ++s) {
+    HY_TRY(stage(m, s, 0));
+    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, m.st, m.ctl, (int)CTL_STAGE_END, s, m.tau, m.evolve, tend);
+    if (m.evolve) m.relax_pending = 1;
+  }
+  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, m.st, m.ctl, (int)CTL_STEP_END, 0, m.tau, m.evolve, tend);
+  HY_HIP(hipGetLastError());
+  return HY_OK;
+}
+
+static int read_ctl(Model& m, Ctl& c) {
+  HY_HIP(hipMemcpyAsync(&c, m.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, m.st));
+  HY_HIP(hipStreamSynchronize(m.st));
+  m.h_t = c.t; m.h_dt = c.dt; m.h_step = c.step;
+  return HY_OK;
+}
+
+// ---------------------------------------------------------- host helpers
+static double interp_linear(const std::vector<double>& x, const std::vector<double>& y, double xq) {
+  size_t n = x.size();  // utils.f90:30-55
+  if (n == 1) return y[0];
+  size_t lo;
+  if (xq <= x[0]) lo = 0;
+  else if (xq >= x[n - 1]) lo = n - 2;
+  else {
+    size_t l = 0, h = n - 1;
+    while (h - l > 1) { size_t mid = (l + h) / 2; if (x[mid] <= xq) l = mid; else h = mid; }
+    lo = l;
+  }
+  double w = (xq - x[lo]) / (x[lo + 1] - x[lo]);
+  return (1.0 - w) * y[lo] + w * y[lo + 1];
+}
+
+// Per-term linear kernels K_i(k) (free_energy.f90:22-59 and each term's kernel)
+static double term_kernel(const Model& m, size_t t, double k) {
+  const hy_term_desc& d = m.terms[t];
+  const double* tab = fd::host_table();
+  const double pi = 3.14159265358979323846264338327950288;
+  switch (d.kind) {
+    case HY_TERM_IDEAL: return 1.0 / (d.beta * d.n0);
+    case HY_TERM_TF: return 2.0 / (d.beta * d.cn * fd::im12(tab, d.alpha0));
+    case HY_TERM_GRADIENT: return 2.0 * d.cg * fd::xi(tab, d.alpha0) * k * k;
+    case HY_TERM_HARTREE: {
+      double den = k * k + d.kappa * d.kappa;
+      return den > 0.0 ? 4.0 * pi * d.e2 / den : 0.0;
+    }
+    case HY_TERM_RY: {
+      const auto& kt = m.ktabs[t];
+      const auto& ct = m.ctabs[t];
+      double k1 = std::max(k, kt[0]);
+      double c = interp_linear(kt, ct, std::min(k1, kt.back()));
+      if (d.subtract_mean_field) c = c + d.beta * 4.0 * pi * d.e2 / (k1 * k1 + d.kappa * d.kappa);
+      return -c / d.beta;
+    }
+  }
+  return 0.0;
+}
+static double fun_kernel(const Model& m, double k) {
+  double kk = 0.0;
+  for (size_t t = 0; t < m.terms.size(); ++t) kk += term_kernel(m, t, k);
+  return kk;
+}
+
+static void model_free(Model& m) {
+  for (void* p : {(void*)m.S, (void*)m.R, (void*)m.uold, (void*)m.kcomb, (void*)m.fext, (void*)m.fdtab,
+                  m.xb, m.zb, m.fb, (void*)m.red, (void*)m.ctl, (void*)m.probe})
+    if (p) (void)hipFree(p);
+  if (m.st) (void)hipStreamDestroy(m.st);
+  grid_free(m.g);
+}
+
+static int dmalloc(void** p, size_t bytes) {
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return set_error(HY_ERR_OOM, "model: device allocation failed");
+  }
+  return HY_OK;
+}
+
+int model_build(Model& m, const hy_model_desc* d);
+
+}  // namespace hy
+
+// =========================================================================== ABI
+using namespace hy;
+
+struct hy_model_s {
+  Model m;
+};
+
+namespace hy {
+int fft_r2c_full_api(Grid& g, const double* u, c64* uhat, hipStream_t s);
+int fft_c2r_full_api(Grid& g, const c64* uhat, double* u, hipStream_t s);
+int spec_grad_api(Grid& g, const double* u, double* du3, hipStream_t s);
+
+int model_build(Model& m, const hy_model_desc* d) {
+  HY_TRY(grid_build(m.g, d->nx, d->ny, d->nz, d->lx, d->ly, d->lz, d->dealias, d->precision));
+  const Grid& g = m.g;
+  m.prec = d->precision;
+  m.esz = mixed(m) ? sizeof(c32) : sizeof(c64);
+  m.mass = d->mass;
+  m.n0 = d->n0;
+  if (!(m.mass > 0.0) || !(m.n0 > 0.0)) return set_error(HY_ERR_INVALID, "model: mass and n0 must be > 0");
+  if (d->nterms < 0 || (d->nterms > 0 && !d->terms)) return set_error(HY_ERR_INVALID, "model: terms");
+  m.terms.assign(d->terms, d->terms + d->nterms);
+  m.ktabs.resize(d->nterms);
+  m.ctabs.resize(d->nterms);
+  for (int t = 0; t < d->nterms; ++t) {
+    const hy_term_desc& td = d->terms[t];
+    switch (td.kind) {
+      case HY_TERM_IDEAL: m.ideal = 1; m.beta = td.beta; break;
+      case HY_TERM_TF: m.tf = 1; m.beta = td.beta; m.cn = td.cn; m.alpha0 = td.alpha0; break;
+      case HY_TERM_GRADIENT: m.grad = 1; m.beta = td.beta; m.cn = td.cn; m.alpha0 = td.alpha0; m.cg = td.cg; break;
+      case HY_TERM_HARTREE: m.hartree = 1; break;
+      case HY_TERM_RY:
+        m.ry = 1;
+        if (td.ntab < 1 || !td.k_tab || !td.c_tab) return set_error(HY_ERR_INVALID, "ry term needs a structure table");
+        m.ktabs[t].assign(td.k_tab, td.k_tab + td.ntab);
+        m.ctabs[t].assign(td.c_tab, td.c_tab + td.ntab);
+        break;
+      default: return set_error(HY_ERR_UNKNOWN_KIND, "unknown free-energy term kind");
+    }
+    if ((td.kind == HY_TERM_TF || td.kind == HY_TERM_GRADIENT) && !(td.cn > 0.0))
+      return set_error(HY_ERR_INVALID, "tf/gradient term needs cn > 0 (kind = electron)");
+  }
+  for (auto& td : m.terms)
+    if (td.kind != HY_TERM_HARTREE && td.kind != HY_TERM_RY) m.beta = td.beta;
+  if (d->closure == HY_CLOSURE_NEWTONIAN) { m.evolve = 0; m.tau = 0.0; }
+  else if (d->closure == HY_CLOSURE_MAXWELL) { m.evolve = d->tau > 0.0; m.tau = d->tau; }
+  else return set_error(HY_ERR_UNKNOWN_KIND, "unknown stress closure");
+  m.closure = d->closure;
+  m.eta = d->eta; m.xi = d->xi; m.eta_l = d->eta_l;
+  m.stress_newt = !m.evolve && (m.eta != 0.0 || m.xi != 0.0);
+  m.vext_kind = d->vext_kind;
+  if (m.vext_kind < 0 || m.vext_kind > 3) return set_error(HY_ERR_UNKNOWN_KIND, "unknown external potential");
+  m.vext_td = m.vext_kind == HY_VEXT_DRIVEN;
+  m.vamp = d->vext_amp; m.vomega = d->vext_omega; m.vwidth = d->vext_width; m.vramp = d->vext_ramp;
+  const double twopi = 2.0 * 3.14159265358979323846264338327950288;
+  for (int i = 0; i < 3; ++i) m.vmode[i] = d->vext_mode[i];
+  m.kvec[0] = twopi * m.vmode[0] / g.lx;  // external_potential.f90:46-48
+  m.kvec[1] = g.ny == 1 ? 0.0 : twopi * m.vmode[1] / g.ly;
+  m.kvec[2] = g.nz == 1 ? 0.0 : twopi * m.vmode[2] / g.lz;
+  m.dt_fixed = d->dt;
+  m.cfl = d->cfl;
+  HY_HIP(hipStreamCreateWithFlags(&m.st, hipStreamNonBlocking));
+  // sizes and buffers
+  m.nt = g.ntrunc();
+  m.xsz = (size_t)g.Kx * g.ny * g.nz;
+  m.zsz = (size_t)g.Kx * g.Ky * g.nz;
+  m.nxslots = m.evolve ? 37 : 17;
+  m.nzslots = 24;
+  m.nfslots = 20;
+  HY_TRY(dmalloc((void**)&m.S, sizeof(c64) * m.nt * 10));
+  HY_TRY(dmalloc((void**)&m.R, sizeof(c64) * m.nt * 10));
+  HY_TRY(dmalloc((void**)&m.uold, sizeof(c64) * m.nt * 3));
+  HY_TRY(dmalloc(&m.xb, m.esz * m.xsz * m.nxslots));
+  HY_TRY(dmalloc(&m.zb, m.esz * m.zsz * m.nzslots));
+  // FS slots in the compute type plus 10 FP64 rhs-output slots
+  HY_TRY(dmalloc(&m.fb, m.esz * m.nt * m.nfslots + sizeof(c64) * m.nt * 10));
+  HY_TRY(dmalloc((void**)&m.red, sizeof(double) * std::max<size_t>((size_t)g.ny * g.nz * 4, 4096)));
+  HY_TRY(dmalloc((void**)&m.ctl, sizeof(Ctl)));
+  HY_HIP(hipMemsetAsync(m.S, 0, sizeof(c64) * m.nt * 10, m.st));
+  HY_HIP(hipMemsetAsync(m.R, 0, sizeof(c64) * m.nt * 10, m.st));
+  HY_HIP(hipMemsetAsync(m.uold, 0, sizeof(c64) * m.nt * 3, m.st));
+  HY_HIP(hipMemsetAsync(m.xb, 0, m.esz * m.xsz * m.nxslots, m.st));
+  HY_HIP(hipMemsetAsync(m.fb, 0, m.esz * m.nt * m.nfslots + sizeof(c64) * m.nt * 10, m.st));
+  Ctl c;
+  std::memset(&c, 0, sizeof(c));
+  c.dt = m.dt_fixed;
+  HY_HIP(hipMemcpyAsync(m.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, m.st));
+  m.h_dt = m.dt_fixed;
+  // state n = n0, mom = 0 (species.f90:162-163)
+  double n0v = m.n0;
+  HY_HIP(hipMemcpyAsync(m.S, &n0v, sizeof(double), hipMemcpyHostToDevice, m.st));
+  // FD table
+  if (m.tf || m.grad) {
+    const double* tab = fd::host_table();
+    HY_TRY(dmalloc((void**)&m.fdtab, sizeof(double) * fd::NTAB * fd::NC));
+    HY_HIP(hipMemcpy(m.fdtab, tab, sizeof(double) * fd::NTAB * fd::NC, hipMemcpyHostToDevice));
+  }
+  // combined convolution kernel on the kept box, and compute_dt constants
+  std::vector<double> kc(m.nt, 0.0);
+  double wmax = 0.0;
+  for (size_t e = 0; e < m.nt; ++e) {
+    int ix = (int)(e % g.Kx), ty = (int)((e / g.Kx) % g.Ky), tz = (int)(e / ((size_t)g.Kx * g.Ky));
+    int jy = ty <= g.My ? ty : ty - g.Ky + g.ny;
+    int jz = tz <= g.Mz ? tz : tz - g.Kz + g.nz;
+    double k2 = g.kx[ix] * g.kx[ix] + g.ky[jy] * g.ky[jy] + g.kz[jz] * g.kz[jz];
+    double kabs = std::sqrt(k2);
+    double v = 0.0;
+    for (size_t t = 0; t < m.terms.size(); ++t) {
+      int kind = m.terms[t].kind;
+      if (kind == HY_TERM_HARTREE || kind == HY_TERM_RY) v += term_kernel(m, t, kabs);
+    }
+    kc[e] = v;
+    if (kabs != 0.0) {  // model.f90:159-168
+      double c2 = m.n0 * fun_kernel(m, kabs) / m.mass;
+      if (m.evolve) c2 = c2 + m.eta_l / (m.mass * m.tau);
+      double kk = kabs * kabs * std::max(c2, 0.0);
+      wmax = std::max(wmax, std::sqrt(kk));
+    }
+  }
+  m.wmax = wmax;
+  m.numax = m.evolve ? 0.0 : m.eta_l / m.mass * g.kmax_dealiased * g.kmax_dealiased;
+  m.dxmin = g.dx;
+  if (g.ny > 1) m.dxmin = std::min(m.dxmin, g.dy);
+  if (g.nz > 1) m.dxmin = std::min(m.dxmin, g.dz);
+  HY_TRY(dmalloc((void**)&m.kcomb, sizeof(double) * m.nt));
+  HY_HIP(hipMemcpy(m.kcomb, kc.data(), sizeof(double) * m.nt, hipMemcpyHostToDevice));
+  // static gaussian force -grad(v) (external_potential.f90:84-91)
+  if (m.vext_kind == HY_VEXT_GAUSSIAN) {
+    std::vector<double> v(g.nreal());
+    for (int k = 0; k < g.nz; ++k)
+      for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nx; ++i) {
+          double x = i * g.dx - 0.5 * g.lx;
+          double y = g.ny > 1 ? j * g.dy - 0.5 * g.ly : 0.0;
+          double z = g.nz > 1 ? k * g.dz - 0.5 * g.lz : 0.0;
+          double r2 = x * x + y * y + z * z;
+          v[((size_t)k * g.ny + j) * g.nx + i] = -m.vamp * std::exp(-0.5 * r2 / (m.vwidth * m.vwidth));
+        }
+    double* dv;
+    HY_TRY(dmalloc((void**)&m.fext, sizeof(double) * g.nreal() * 3));
+    HY_TRY(dmalloc((void**)&dv, sizeof(double) * g.nreal()));
+    HY_HIP(hipMemcpy(dv, v.data(), sizeof(double) * g.nreal(), hipMemcpyHostToDevice));
+    int rc = spec_grad_api(m.g, dv, m.fext, m.st);
+    HY_HIP(hipStreamSynchronize(m.st));
+    (void)hipFree(dv);
+    HY_TRY(rc);
+    std::vector<double> f(g.nreal() * 3);
+    HY_HIP(hipMemcpy(f.data(), m.fext, sizeof(double) * f.size(), hipMemcpyDeviceToHost));
+    for (auto& x : f) x = -x;
+    HY_HIP(hipMemcpy(m.fext, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice));
+  }
+  HY_HIP(hipStreamSynchronize(m.st));
+  return HY_OK;
+}
+}  // namespace hy
+
+extern "C" {
+
+int hy_model_create(const hy_model_desc* d, hy_model* out) {
+  if (!d || !out) return set_error(HY_ERR_INVALID, "hy_model_create: null argument");
+  auto* h = new hy_model_s();
+  int rc = model_build(h->m, d);
+  if (rc != HY_OK) {
+    model_free(h->m);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return HY_OK;
+}
+
+int hy_model_destroy(hy_model h) {
+  if (!h) return HY_OK;
+  (void)hipStreamSynchronize(h->m.st);
+  model_free(h->m);
+  delete h;
+  return HY_OK;
+}
+
+// Real fields -> kept-box spectra, in FP64 (the state is FP64 in every mode).
+int hy_model_set_state(hy_model h, const double* h_n, const double* h_mom3, const double* h_pi6) {
+  if (!h || !h_n) return set_error(HY_ERR_INVALID, "null argument");
+  Model& m = h->m;
+  Grid& g = m.g;
+  const size_t nr = g.nreal();
+  double* dr;
+  c64* df;
+  HY_TRY(dmalloc((void**)&dr, sizeof(double) * nr));
+  int rc = dmalloc((void**)&df, sizeof(c64) * g.nfull());
+  if (rc) { (void)hipFree(dr); return rc; }
+  SpecArgs sa = spec_args(m);
+  auto put = [&](const double* src, int f) -> int {
+    HY_HIP(hipMemcpyAsync(dr, src, sizeof(double) * nr, hipMemcpyHostToDevice, m.st));
+    HY_TRY(fft_r2c_full_api(g, dr, df, m.st));
+    hipLaunchKernelGGL(k_full_to_trunc, dim3(nblocks(m.nt)), dim3(256), 0, m.st, df, sfield(m, f), sa, g.nkx);
+    HY_HIP(hipGetLastError());
+    return HY_OK;
+  };
+  rc = put(h_n, 0);
+  for (int d = 0; d < 3 && rc == HY_OK; ++d) {
+    if (h_mom3) rc = put(h_mom3 + nr * d, 1 + d);
+    else rc = hipMemsetAsync(sfield(m, 1 + d), 0, sizeof(c64) * m.nt, m.st) == hipSuccess ? HY_OK : HY_ERR_HIP;
+  }
+  if (m.evolve)
+    for (int c = 0; c < 6 && rc == HY_OK; ++c) {
+      if (h_pi6) rc = put(h_pi6 + nr * c, 4 + c);
+      else rc = hipMemsetAsync(sfield(m, 4 + c), 0, sizeof(c64) * m.nt, m.st) == hipSuccess ? HY_OK : HY_ERR_HIP;
+    }
+  (void)hipStreamSynchronize(m.st);
+  (void)hipFree(dr);
+  (void)hipFree(df);
+  m.relax_pending = 0;
+  return rc;
+}
+
+static int trunc_to_real(Model& m, const c64* tr, double* h_out) {
+  Grid& g = m.g;
+  const size_t nr = g.nreal();
+  double* dr;
+  c64* df;
+  HY_TRY(dmalloc((void**)&dr, sizeof(double) * nr));
+  int rc = dmalloc((void**)&df, sizeof(c64) * g.nfull());
+  if (rc) { (void)hipFree(dr); return rc; }
+  SpecArgs sa = spec_args(m);
+  hipLaunchKernelGGL(k_trunc_to_full, dim3(nblocks(g.nfull())), dim3(256), 0, m.st, tr, df, sa, g.nkx);
+  rc = fft_c2r_full_api(g, df, dr, m.st);
+  if (rc == HY_OK && hipMemcpyAsync(h_out, dr, sizeof(double) * nr, hipMemcpyDeviceToHost, m.st) != hipSuccess)
+    rc = set_error(HY_ERR_HIP, "copy");
+  if (hipStreamSynchronize(m.st) != hipSuccess) rc = set_error(HY_ERR_HIP, "sync");
+  (void)hipFree(dr);
+  (void)hipFree(df);
+  return rc;
+}
+
+int hy_model_get_state(hy_model h, double* h_n, double* h_mom3, double* h_pi6) {
+  if (!h) return set_error(HY_ERR_INVALID, "null argument");
+  Model& m = h->m;
+  const size_t nr = m.g.nreal();
+  HY_TRY(finish_relax(m));
+  if (h_n) HY_TRY(trunc_to_real(m, sfield(m, 0), h_n));
+  if (h_mom3)
+    for (int d = 0; d < 3; ++d) HY_TRY(trunc_to_real(m, sfield(m, 1 + d), h_mom3 + nr * d));
+  if (h_pi6) {
+    if (!m.evolve) return set_error(HY_ERR_INVALID, "get_state: no stress state (newtonian closure)");
+    for (int c = 0; c < 6; ++c) HY_TRY(trunc_to_real(m, sfield(m, 4 + c), h_pi6 + nr * c));
+  }
+  return HY_OK;
+}
+
+int hy_model_set_time(hy_model h, double t, int step) {
+  if (!h) return set_error(HY_ERR_INVALID, "null argument");
+  Model& m = h->m;
+  Ctl c;
+  HY_TRY(read_ctl(m, c));
+  c.t = t; c.ts = t; c.step = step;
+  HY_HIP(hipMemcpyAsync(m.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, m.st));
+  HY_HIP(hipStreamSynchronize(m.st));
+  m.h_t = t; m.h_step = step;
+  return HY_OK;
+}
+
+int hy_model_get_time(hy_model h, double* t, double* dt, int* step) {
+  if (!h) return set_error(HY_ERR_INVALID, "null argument");
+  Ctl c;
+  HY_TRY(read_ctl(h->m, c));
+  if (t) *t = c.t;
+  if (dt) *dt = c.dt;
+  if (step) *step = c.step;
+  return HY_OK;
+}
+
+int hy_model_set_dt(hy_model h, double dt) {
+  if (!h) return set_error(HY_ERR_INVALID, "null argument");
+  Model& m = h->m;
+  HY_HIP(hipMemcpyAsync(&m.ctl->dt, &dt, sizeof(double), hipMemcpyHostToDevice, m.st));
+  HY_HIP(hipStreamSynchronize(m.st));
+  m.h_dt = dt;
+  return HY_OK;
+}
+
+int hy_model_compute_dt(hy_model h, double* h_dt) {
+  if (!h) return set_error(HY_ERR_INVALID, "null argument");
+  Model& m = h->m;
+  HY_TRY(finish_relax(m));
+  HY_TRY(compute_dt_dev(m));
+  Ctl c;
+  HY_TRY(read_ctl(m, c));
+  if (c.nodt) return set_error(HY_ERR_NO_DT, "cannot determine a time step; set dt in &time");
+  if (h_dt) *h_dt = c.dt;
+  return HY_OK;
+}
+
+int hy_model_advance(hy_model h, int nsteps, int dt_every, double tend, const int* probe3,
+                     double* h_probe, int* h_taken) {
+  if (!h || nsteps < 0) return set_error(HY_ERR_INVALID, "hy_model_advance: bad argument");
+  Model& m = h->m;
+  const Grid& g = m.g;
+  long long pidx = -1;
+  if (probe3) {
+    if (!h_probe) return set_error(HY_ERR_INVALID, "probe buffer missing");
+    int mx = probe3[0], my = probe3[1], mz = probe3[2];
+    if (mx < 0 || mx >= g.nkx) return set_error(HY_ERR_INVALID, "probe: mx out of range");
+    int wy = ((my % g.ny) + g.ny) % g.ny, wz = ((mz % g.nz) + g.nz) % g.nz;
+    int sy = wy <= g.ny / 2 ? wy : wy - g.ny, sz = wz <= g.nz / 2 ? wz : wz - g.nz;
+    if (mx < g.Kx && std::abs(sy) <= g.My && std::abs(sz) <= g.Mz) {
+      int ty = sy >= 0 ? sy : sy + g.Ky, tz = sz >= 0 ? sz : sz + g.Kz;
+      pidx = ((long long)tz * g.Ky + ty) * g.Kx + mx;
+    }
+    if (m.probe_cap < nsteps) {
+      if (m.probe) (void)hipFree(m.probe);
+      HY_TRY(dmalloc((void**)&m.probe, sizeof(double) * 6 * std::max(nsteps, 1)));
+      m.probe_cap = nsteps;
+    }
+  }
+  int taken = 0;
+  for (int i = 0; i < nsteps; ++i) {
+    if (tend > 0.0) {
+      Ctl c;
+      HY_TRY(read_ctl(m, c));
+      if (c.t >= tend) break;
+    }
+    if (dt_every > 0) {
+      if (m.irecompute == 0) {
+        HY_TRY(finish_relax(m));
+        HY_TRY(compute_dt_dev(m));
+        if (tend > 0.0 || i == 0) {
+          Ctl c;
+          HY_TRY(read_ctl(m, c));
+          if (c.nodt) return set_error(HY_ERR_NO_DT, "cannot determine a time step; set dt in &time");
+        }
+      }
+      m.irecompute = (m.irecompute + 1) % dt_every;
+    }
+    HY_TRY(step_once(m, tend));
+    if (probe3) hipLaunchKernelGGL(k_probe, dim3(1), dim3(1), 0, m.st, m.ctl, m.S, m.nt, pidx, m.n0, m.probe + 6 * (size_t)taken);
+    ++taken;
+  }
+  HY_TRY(finish_relax(m));
+  Ctl c;
+  HY_TRY(read_ctl(m, c));
+  if (probe3 && taken > 0) {
+    HY_HIP(hipMemcpy(h_probe, m.probe, sizeof(double) * 6 * taken, hipMemcpyDeviceToHost));
+  }
+  if (h_taken) *h_taken = taken;
+  if (c.nonpos) return set_error(HY_ERR_NONPOSITIVE, "density became non-positive; reduce the amplitude or the time step");
+  if (c.nodt) return set_error(HY_ERR_NO_DT, "cannot determine a time step; set dt in &time");
+  return HY_OK;
+}
+
+int hy_model_rhs(hy_model h, double t, double* h_dn, double* h_dmom3, double* h_dpi6) {
+  if (!h) return set_error(HY_ERR_INVALID, "null argument");
+  Model& m = h->m;
+  HY_TRY(finish_relax(m));
+  Ctl c;
+  HY_TRY(read_ctl(m, c));
+  c.ts = t;
+  HY_HIP(hipMemcpyAsync(m.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice, m.st));
+  HY_TRY(stage(m, 0, 1));
+  const c64* R = reinterpret_cast<const c64*>(fslot(m, FS_RHS));
+  const size_t nr = m.g.nreal();
+  if (h_dn) HY_TRY(trunc_to_real(m, R, h_dn));
+  if (h_dmom3)
+    for (int d = 0; d < 3; ++d) HY_TRY(trunc_to_real(m, R + m.nt * (1 + d), h_dmom3 + nr * d));
+  if (h_dpi6 && m.evolve)
+    for (int cc = 0; cc < 6; ++cc) HY_TRY(trunc_to_real(m, R + m.nt * (4 + cc), h_dpi6 + nr * cc));
+  Ctl c2;
+  HY_TRY(read_ctl(m, c2));
+  if (c2.nonpos) return set_error(HY_ERR_NONPOSITIVE, "density became non-positive");
+  return HY_OK;
+}
+
+int hy_model_probe(hy_model h, int mx, int my, int mz, double* out4) {
+  if (!h || !out4) return set_error(HY_ERR_INVALID, "null argument");
+  Model& m = h->m;
+  const Grid& g = m.g;
+  HY_TRY(finish_relax(m));
+  if (mx < 0 || mx >= g.nkx) return set_error(HY_ERR_INVALID, "probe: mx out of range");
+  int wy = ((my % g.ny) + g.ny) % g.ny, wz = ((mz % g.nz) + g.nz) % g.nz;
+  int sy = wy <= g.ny / 2 ? wy : wy - g.ny, sz = wz <= g.nz / 2 ? wz : wz - g.nz;
+  c64 a = mk<double>(0, 0), b = mk<double>(0, 0);
+  if (mx < g.Kx && std::abs(sy) <= g.My && std::abs(sz) <= g.Mz) {
+    int ty = sy >= 0 ? sy : sy + g.Ky, tz = sz >= 0 ? sz : sz + g.Kz;
+    size_t idx = ((size_t)tz * g.Ky + ty) * g.Kx + mx;
+    HY_HIP(hipMemcpy(&a, m.S + idx, sizeof(c64), hipMemcpyDeviceToHost));
+    HY_HIP(hipMemcpy(&b, m.S + m.nt + idx, sizeof(c64), hipMemcpyDeviceToHost));
+  }
+  out4[0] = a.x / m.n0; out4[1] = a.y / m.n0; out4[2] = b.x; out4[3] = b.y;
+  return HY_OK;
+}
+
+int hy_model_kernel(hy_model h, double k, double* out) {
+  if (!h || !out) return set_error(HY_ERR_INVALID, "null argument");
+  *out = fun_kernel(h->m, k);
+  return HY_OK;
+}
+
+int hy_fd_eval(int which, int n, const double* x, double* y) {
+  if (n < 0 || (n > 0 && (!x || !y))) return set_error(HY_ERR_INVALID, "null argument");
+  const double* tab = fd::host_table();
+  for (int i = 0; i < n; ++i) {
+    double a = x[i];
+    switch (which) {
+      case 0: y[i] = fd::im12(tab, a); break;
+      case 1: y[i] = fd::ip12(tab, a); break;
+      case 2: y[i] = fd::ip32(tab, a); break;
+      case 3: y[i] = fd::dim12(tab, a); break;
+      case 4: y[i] = fd::d2im12(tab, a); break;
+      case 5: y[i] = fd::xi(tab, a); break;
+      case 6: y[i] = fd::dxi(tab, a); break;
+      case 7:
+        if (!(a > 0.0)) return set_error(HY_ERR_INVALID, "fermi_dirac: I_{1/2} inverse needs y > 0");
+        y[i] = fd::alpha_from_ip12(tab, a);
+        break;
+      default: return set_error(HY_ERR_INVALID, "hy_fd_eval: unknown function");
+    }
+  }
+  return HY_OK;
+}
+
+}  // extern "C"

@@ -73,6 +73,11 @@ int hy_spec_mean(hy_grid g, const double* d_u, double* h_mean, void* stream);
 int hy_spec_mode(hy_grid g, const double* d_u, int mx, int my, int mz, double* h_re_im,
                  void* stream);
 
+/* ---- fermi_dirac.f90 functions (host): which = 0 I_{-1/2}, 1 I_{1/2},
+   2 I_{3/2}, 3 I'_{-1/2}, 4 I''_{-1/2}, 5 xi, 6 dxi/dalpha, 7 inverse of I_{1/2}
+   (fermi_dirac.f90:150-267) */
+int hy_fd_eval(int which, int n, const double* h_x, double* h_y);
+
 /* ---- stress_t (src/closure/stress.f90:60-105), code-unit eta, xi -------- */
 int hy_stress_pi0(hy_grid g, double eta, double xi, const double* d_u3, double* d_pi6,
                   void* stream);

@@ -133,7 +133,7 @@ class Spec:
     # fft.f90:62-69: backward unnormalised, input preserved
     def bwd(self, uhat):
         self.nbwd += 1
-        return np.fft.irfftn(uhat, s=self.g.shape, norm="forward")
+        return np.fft.irfftn(uhat, s=self.g.shape, axes=(0, 1, 2), norm="forward")
 
     def grad_hat(self, uhat, idir):
         """spectral.f90:67-90 (idir 0-based)."""

@@ -1,0 +1,83 @@
+"""Pins the Fermi-Dirac functions used by the electron functional (CPU).
+
+The checks of test_fermi_dirac.f90:16-52 (limits, known value I_1/2(0),
+derivative relations, inverse), an independent scipy.integrate.quad of
+int_0^inf x^p/(1+e^(x-alpha)) dx, and agreement between the oracle's numpy
+restatement and libhydft's host implementation (the one the GPU kernels use)."""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from oracle import hydft_oracle as O
+
+fd = O.fermi_dirac()
+SQRTPI = math.sqrt(math.pi)
+
+
+def v(f, a):
+    return float(f(np.array([a]))[0])
+
+
+def test_limits_and_known_value():
+    a = -20.0
+    assert abs(v(fd.ip12, a) / (0.5 * SQRTPI * math.exp(a)) - 1) < 1e-8
+    assert abs(v(fd.im12, a) / (SQRTPI * math.exp(a)) - 1) < 1e-8
+    assert abs(v(fd.ip32, a) / (0.75 * SQRTPI * math.exp(a)) - 1) < 1e-8
+    a = 250.0
+    ex = (2.0 / 3.0) * a ** 1.5 * (1.0 + math.pi ** 2 / (8.0 * a * a))
+    assert abs(v(fd.ip12, a) / ex - 1) < 1e-8
+    ex = 2.0 * math.sqrt(a) * (1.0 - math.pi ** 2 / (24.0 * a * a))
+    assert abs(v(fd.im12, a) / ex - 1) < 1e-8
+    assert abs(v(fd.ip12, 300 - 1e-9) / v(fd.ip12, 300 + 1e-9) - 1) < 1e-8
+    assert abs(v(fd.ip12, -30 - 1e-9) / v(fd.ip12, -30 + 1e-9) - 1) < 1e-8
+    assert abs(v(fd.ip12, 0.0) - 0.678093895) < 1e-8
+
+
+def test_derivative_relations():
+    hh = 1e-4
+    for ia in range(10):
+        a = -5.0 + 4.7 * ia
+        d = (v(fd.ip12, a + hh) - v(fd.ip12, a - hh)) / (2 * hh)
+        assert abs(d / (0.5 * v(fd.im12, a)) - 1) < 1e-7
+        d = (v(fd.ip32, a + hh) - v(fd.ip32, a - hh)) / (2 * hh)
+        assert abs(d / (1.5 * v(fd.ip12, a)) - 1) < 1e-7
+        d = (v(fd.im12, a + hh) - v(fd.im12, a - hh)) / (2 * hh)
+        assert abs(d / v(fd.dim12, a) - 1) < 1e-6
+        d = (v(fd.dim12, a + hh) - v(fd.dim12, a - hh)) / (2 * hh)
+        assert abs(d / v(fd.d2im12, a) - 1) < 1e-5
+        d = (v(fd.xi, a + hh) - v(fd.xi, a - hh)) / (2 * hh)
+        assert abs(d / v(fd.dxi, a) - 1) < 1e-5
+
+
+def test_inverse():
+    for ia in range(23):
+        a = -25.0 + 13.3 * ia
+        y = v(fd.ip12, a)
+        assert abs(v(fd.alpha_from_ip12, y) - a) < 1e-9 * max(1.0, abs(a))
+
+
+@pytest.mark.parametrize("a", [-3.0, 0.0, 1.7, 8.0, 40.0])
+def test_against_quadrature(a):
+    def I(p):
+        f = lambda x: x ** p / (1.0 + math.exp(x - a)) if x - a < 700 else 0.0
+        lim = max(a, 0) + 60
+        r1 = integrate.quad(f, 0, 1, limit=200, epsabs=0, epsrel=1e-13)[0]
+        r2 = integrate.quad(f, 1, lim, limit=400, epsabs=0, epsrel=1e-13)[0]
+        return r1 + r2
+    assert abs(v(fd.ip12, a) / I(0.5) - 1) < 1e-9
+    assert abs(v(fd.ip32, a) / I(1.5) - 1) < 1e-9
+    # I_{-1/2} by parts: I_{-1/2} = 2 dI_{1/2}/dalpha; quad of x^-1/2 with the endpoint singularity
+    im = integrate.quad(lambda t: 2.0 / (1.0 + math.exp(t * t - a)), 0, math.sqrt(max(a, 0) + 60),
+                        limit=400, epsrel=1e-13)[0]
+    assert abs(v(fd.im12, a) / im - 1) < 1e-9
+
+
+def test_library_host_fd_matches_oracle():
+    from hydft_amd.model import fd_eval
+    xs = np.array([-35.0, -29.99, -12.3, -0.7, 0.0, 0.33, 4.4, 77.7, 299.9, 301.5])
+    for which, f in enumerate([fd.im12, fd.ip12, fd.ip32, fd.dim12, fd.d2im12, fd.xi, fd.dxi]):
+        np.testing.assert_allclose(fd_eval(which, xs), f(xs), rtol=1e-13, atol=0)
+    ys = fd.ip12(xs)
+    np.testing.assert_allclose(fd_eval(7, ys), fd.alpha_from_ip12(ys), rtol=1e-12, atol=1e-12)
