@@ -106,6 +106,8 @@ _SIGS = {
     "hy_model_probe": [_P, _I, _I, _I, _P],
     "hy_model_kernel": [_P, _D, _PD],
     "hy_fd_eval": [_I, _I, _P, _P],
+    "hy_model_profile": [_P, _I],
+    "hy_model_profile_read": [_P, _I, _P, _P, _P],
 }
 
 
@@ -113,7 +115,7 @@ def declared_symbols():
     """Every function name declared in include/hydft.h."""
     import re
     hdr = Path(__file__).resolve().parent.parent / "include" / "hydft.h"
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(hy_\w+)\s*\(", hdr.read_text(), re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|const char\s*\*)\s*(hy_\w+)\s*\(", hdr.read_text(), re.M)))
 
 
 class _Lib:
@@ -123,6 +125,8 @@ class _Lib:
         self.lib = C.CDLL(str(path))
         self.lib.hy_last_error.restype = C.c_char_p
         self.lib.hy_last_error.argtypes = []
+        self.lib.hy_profile_class_name.restype = C.c_char_p
+        self.lib.hy_profile_class_name.argtypes = [C.c_int]
         self.missing = []
         for name, args in _SIGS.items():
             try:
@@ -135,6 +139,8 @@ class _Lib:
 
     def call(self, name, *args):
         rc = getattr(self.lib, name)(*args)
+        if name == "hy_model_profile_read" and rc > 0:
+            return rc
         if rc != HY_OK:
             raise HydftError(rc, self.lib.hy_last_error().decode())
         return rc

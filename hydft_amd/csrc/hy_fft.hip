@@ -2,6 +2,8 @@
 // spectral operators (fft_t, spec_t, stress_t of the reference) on FP64
 // device buffers in the reference layouts.
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -14,6 +16,26 @@ static thread_local std::string g_last_error;
 int set_error(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
+}
+
+static int trace_level() {
+  static int lv = -1;
+  if (lv < 0) {
+    const char* e = std::getenv("HYDFT_TRACE");
+    lv = (e && e[0] == '1') ? 1 : 0;
+  }
+  return lv;
+}
+int trace_launch(const char* what, hipStream_t s) {
+  if (!trace_level()) return HY_OK;
+  std::fprintf(stderr, "[hydft] launched %s\n", what);
+  std::fflush(stderr);
+  hipError_t e = hipStreamSynchronize(s);
+  if (e == hipSuccess) e = hipGetLastError();
+  std::fprintf(stderr, "[hydft] done %s: %s\n", what, hipGetErrorString(e));
+  std::fflush(stderr);
+  if (e != hipSuccess) return set_error(HY_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return HY_OK;
 }
 
 // ------------------------------------------------------------------- grid
@@ -237,9 +259,10 @@ __global__ __launch_bounds__(NT) void k_pencil_fwd(PencilArgs a, const cpx<T>* _
   }
 }
 
+// Tile of NA pencils per block, at most 16 KB of LDS (1024 c64 or 2048 c32).
 template <class T>
 constexpr int pencil_na(int L) {
-  int na = (sizeof(T) == 16 ? 2048 : 4096) / L;
+  int na = (sizeof(T) == 16 ? 1024 : 2048) / L;
   return na < 4 ? 4 : (na > 64 ? 64 : na);
 }
 template <class T>
@@ -267,6 +290,7 @@ static int dispatch_pencil(const PencilArgs& a, bool inverse, int nf, const cpx<
     default: return set_error(HY_ERR_UNSUPPORTED, "pencil: unsupported length");
   }
   HY_HIP(hipGetLastError());
+  HY_TRACE(inverse ? "pencil_inverse" : "pencil_forward", s);
   return HY_OK;
 }
 
@@ -370,6 +394,7 @@ static int xplain(const Grid& g, bool fwd, const void* in, void* out, int K, hip
     default: return set_error(HY_ERR_UNSUPPORTED, "x pass: unsupported length");
   }
   HY_HIP(hipGetLastError());
+  HY_TRACE(fwd ? "xplain_r2c" : "xplain_c2r", s);
   return HY_OK;
 }
 
@@ -450,6 +475,7 @@ static int specop(const Grid& g, int op, int dir, const c64* in, const double* k
   int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
   hipLaunchKernelGGL(k_specop, dim3(blocks), dim3(256), 0, s, full_args(g), op, dir, in, kern, out);
   HY_HIP(hipGetLastError());
+  HY_TRACE("specop", s);
   return HY_OK;
 }
 
@@ -716,22 +742,23 @@ __global__ void k_pi0_combo(FullSpecArgs a, int comp, double eta, double xi,
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     int i = e % a.nkx, j = (e / a.nkx) % a.ny, k = e / ((size_t)a.nkx * a.ny);
     double m = full_kept(a, i, j, k) ? 1.0 : 0.0;
-    double kv[3] = {a.kx[i], a.ky[j], a.kz[k]};
-    c64 uh[3] = {ndim > 0 ? u0[e] : mk<double>(0, 0), ndim > 1 ? u1[e] : mk<double>(0, 0),
-                 ndim > 2 ? u2[e] : mk<double>(0, 0)};
-    // g[i][j] = i k_j uhat_i (spectrum of du_i/dx_j)
-    auto g = [&](int ii, int jj) { c64 v = uh[ii]; return mk<double>(-kv[jj] * v.y, kv[jj] * v.x); };
-    c64 divu = g(0, 0) + g(1, 1) + g(2, 2);
+    const double k0 = a.kx[i], k1 = a.ky[j], k2 = a.kz[k];
+    const c64 z = mk<double>(0, 0);
+    const c64 v0 = ndim > 0 ? u0[e] : z, v1 = ndim > 1 ? u1[e] : z, v2 = ndim > 2 ? u2[e] : z;
+    // spectrum of du_i/dx_j is i k_j uhat_i
+    const c64 g00 = mk<double>(-k0 * v0.y, k0 * v0.x), g11 = mk<double>(-k1 * v1.y, k1 * v1.x),
+              g22 = mk<double>(-k2 * v2.y, k2 * v2.x);
+    const c64 divu = g00 + g11 + g22;
     c64 r;
     if (comp < 3) {
-      c64 gii = g(comp, comp);
+      const c64 gii = comp == 0 ? g00 : (comp == 1 ? g11 : g22);
       r = scal(scal(gii, 2.0) - scal(divu, 2.0 / 3.0), eta) + scal(divu, xi);
-    } else if (comp == 3) {
-      r = scal(g(0, 1) + g(1, 0), eta);
-    } else if (comp == 4) {
-      r = scal(g(0, 2) + g(2, 0), eta);
-    } else {
-      r = scal(g(1, 2) + g(2, 1), eta);
+    } else if (comp == 3) {  // g01 + g10
+      r = scal(mk<double>(-k1 * v0.y, k1 * v0.x) + mk<double>(-k0 * v1.y, k0 * v1.x), eta);
+    } else if (comp == 4) {  // g02 + g20
+      r = scal(mk<double>(-k2 * v0.y, k2 * v0.x) + mk<double>(-k0 * v2.y, k0 * v2.x), eta);
+    } else {                 // g12 + g21
+      r = scal(mk<double>(-k2 * v1.y, k2 * v1.x) + mk<double>(-k1 * v2.y, k1 * v2.x), eta);
     }
     out[e] = scal(r, m);
   }

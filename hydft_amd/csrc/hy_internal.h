@@ -15,6 +15,10 @@ namespace hy {
 
 // ------------------------------------------------------------------ errors
 int set_error(int code, const std::string& msg);
+// HYDFT_TRACE=1: synchronise after every launch and print its name to stderr
+// (debugging aid; off by default).
+int trace_launch(const char* what, hipStream_t s);
+#define HY_TRACE(what, s) HY_TRY(::hy::trace_launch(what, s))
 #define HY_HIP(call)                                                               \
   do {                                                                             \
     hipError_t e_ = (call);                                                        \

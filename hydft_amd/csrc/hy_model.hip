@@ -97,6 +97,7 @@ struct Ctl {
   double relax_fac, relax_phi;
   double nf[3];              // mean(n f_ext,i) of the current stage
   double umax;
+  double t_vext;             // time of the last v_ext evaluation (stage start)
   int step;
   int nonpos;                // set when a non-positive density is seen
   int nodt;                  // compute_dt could not find a time step
@@ -114,6 +115,7 @@ __global__ void k_ctl(Ctl* c, int op, int s, double tau, int evolve, double tend
     if (tend > 0.0 && c->t + c->dt > tend) c->dt = tend - c->t;  // src/hydft.f90:44
     c->ts = c->t;
   } else if (op == CTL_STAGE_END) {
+    c->t_vext = c->ts;
     double dts = (c_rk_a[s] + c_rk_b[s]) * c->dt;
     c->ts = c->ts + dts;
     if (evolve) {
@@ -754,7 +756,33 @@ struct Model {
   // host mirror of the time
   double h_t = 0, h_dt = 0;
   int h_step = 0;
+  // per-kernel-class profiling (HIP events on the model stream)
+  bool prof = false;
+  struct Ev { int cls; hipEvent_t a, b; double bytes; };
+  std::vector<Ev> evs;
+  std::vector<hipEvent_t> pool;
 };
+
+enum ProfCls { PC_ZINV = 0, PC_YINV, PC_XA, PC_XB, PC_YFWD, PC_ZFWD, PC_K1, PC_K2, PC_UMAX, PC_MISC, PC_N };
+static const char* prof_names[PC_N] = {"pencil_z_inverse", "pencil_y_inverse", "x_phaseA", "x_phaseB",
+                                       "pencil_y_forward", "pencil_z_forward", "spec_mu", "spec_update",
+                                       "x_umax", "misc"};
+static hipEvent_t prof_event(Model& m) {
+  if (!m.pool.empty()) { hipEvent_t e = m.pool.back(); m.pool.pop_back(); return e; }
+  hipEvent_t e;
+  (void)hipEventCreate(&e);
+  return e;
+}
+static void prof_begin(Model& m, int cls, double bytes) {
+  if (!m.prof) return;
+  Model::Ev ev{cls, prof_event(m), prof_event(m), bytes};
+  (void)hipEventRecord(ev.a, m.st);
+  m.evs.push_back(ev);
+}
+static void prof_end(Model& m) {
+  if (!m.prof || m.evs.empty()) return;
+  (void)hipEventRecord(m.evs.back().b, m.st);
+}
 
 static bool mixed(const Model& m) { return m.prec == HY_PREC_MIXED; }
 static void* xslot(Model& m, int i) { return (char*)m.xb + m.xsz * m.esz * i; }
@@ -787,6 +815,13 @@ static int FS_B(int o) { return 10 + o; }     // phase B outputs, 10..18
 static const int FS_MU = 19;
 static const int FS_RHS = 20;                 // rhs-only output: 10 slots of c64 (20..)
 
+static double pencil_bytes(const Model& m, const std::vector<PencilField>& f, size_t A, size_t B, size_t L,
+                           size_t K) {
+  double b = 0.0;
+  for (auto& x : f) b += (double)A * B * (K * (x.in_f64 ? 16.0 : (double)m.esz) + L * (double)m.esz);
+  return b;
+}
+
 // ---- inverse chains: truncated spectrum -> x-line slot (returns the XIn)
 struct InvSpec {
   const void* src;
@@ -808,8 +843,14 @@ static int inv_chain(Model& m, const std::vector<InvSpec>& fs, std::vector<XIn>&
       yf.push_back({zslot(m, (int)i), xslot(m, f.xslot), f.my, 0, 0.0});
     }
     if ((int)fs.size() > m.nzslots) return set_error(HY_ERR_INVALID, "internal: z slots");
+    prof_begin(m, PC_ZINV, pencil_bytes(m, zf, (size_t)g.Kx * g.Ky, 1, g.nz, g.Kz));
     HY_TRY(pencil_pass(g, 2, true, true, g.Kx * g.Ky, 1, zf.data(), (int)zf.size(), m.st));
-    if (g.ny > 1) HY_TRY(pencil_pass(g, 1, true, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st));
+    prof_end(m);
+    if (g.ny > 1) {
+      prof_begin(m, PC_YINV, pencil_bytes(m, yf, g.Kx, g.nz, g.ny, g.Ky));
+      HY_TRY(pencil_pass(g, 1, true, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st));
+      prof_end(m);
+    }
     for (auto& f : fs) xin.push_back({xslot(m, f.xslot), 0, f.mx, 0.0});
     // ny == 1 with nz > 1: the z output is already (Kx, 1, nz) x-lines
     if (g.ny == 1)
@@ -817,7 +858,9 @@ static int inv_chain(Model& m, const std::vector<InvSpec>& fs, std::vector<XIn>&
   } else if (g.ny > 1) {
     std::vector<PencilField> yf;
     for (auto& f : fs) yf.push_back({f.src, xslot(m, f.xslot), f.my, f.src64, f.k0off});
+    prof_begin(m, PC_YINV, pencil_bytes(m, yf, g.Kx, 1, g.ny, g.Ky));
     HY_TRY(pencil_pass(g, 1, true, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st));
+    prof_end(m);
     for (auto& f : fs) xin.push_back({xslot(m, f.xslot), 0, f.mx, 0.0});
   } else {
     for (auto& f : fs) xin.push_back({f.src, f.src64, f.mx, f.k0off});
@@ -842,12 +885,20 @@ static int fwd_chain(Model& m, const std::vector<std::pair<int, int>>& f /* (xsl
       zf.push_back({zin, fslot(m, f[i].second), MUL_NONE, 0, 0.0});
     }
     if ((int)f.size() > m.nzslots) return set_error(HY_ERR_INVALID, "internal: z slots");
-    if (g.ny > 1) HY_TRY(pencil_pass(g, 1, false, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st));
+    if (g.ny > 1) {
+      prof_begin(m, PC_YFWD, pencil_bytes(m, yf, g.Kx, g.nz, g.ny, g.Ky));
+      HY_TRY(pencil_pass(g, 1, false, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st));
+      prof_end(m);
+    }
+    prof_begin(m, PC_ZFWD, pencil_bytes(m, zf, (size_t)g.Kx * g.Ky, 1, g.nz, g.Kz));
     HY_TRY(pencil_pass(g, 2, false, true, g.Kx * g.Ky, 1, zf.data(), (int)zf.size(), m.st));
+    prof_end(m);
   } else if (g.ny > 1) {
     std::vector<PencilField> yf;
     for (auto& p : f) yf.push_back({xslot(m, p.first), fslot(m, p.second), MUL_NONE, 0, 0.0});
+    prof_begin(m, PC_YFWD, pencil_bytes(m, yf, g.Kx, 1, g.ny, g.Ky));
     HY_TRY(pencil_pass(g, 1, false, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st));
+    prof_end(m);
   }
   return HY_OK;
 }
@@ -871,12 +922,31 @@ static int dispatch_xk(int which, const Model& m, const PhysArgs& a, const cpx<T
     default: return set_error(HY_ERR_UNSUPPORTED, "x kernel: unsupported nx");
   }
   HY_HIP(hipGetLastError());
+  HY_TRACE(which == XK_A ? "x_phaseA" : (which == XK_B ? "x_phaseB" : "x_umax"), m.st);
   return HY_OK;
+}
+static double xk_bytes(const Model& m, const PhysArgs& a) {
+  std::vector<const void*> seen;
+  double b = 0.0;
+  const double line = (double)m.g.Kx * m.g.ny * m.g.nz;
+  auto in = [&](const XIn& x) {
+    if (!x.p) return;
+    for (auto q : seen) if (q == x.p) return;
+    seen.push_back(x.p);
+    b += line * (x.is64 ? 16.0 : (double)m.esz);
+  };
+  in(a.n); for (int d = 0; d < 3; ++d) in(a.m[d]);
+  in(a.gx); in(a.gy); in(a.gz); in(a.lap); in(a.mux); in(a.muy); in(a.muz);
+  for (int c = 0; c < 6; ++c) { in(a.px[c]); in(a.py[c]); in(a.pz[c]); }
+  b += line * a.nout * (double)m.esz;
+  return b;
 }
 static int run_xk(Model& m, int which, const PhysArgs& a) {
   const TwiddleSet& t = m.g.tw(m.g.nx);
-  if (mixed(m)) return dispatch_xk<float>(which, m, a, t.d32);
-  return dispatch_xk<double>(which, m, a, t.d64);
+  prof_begin(m, which == XK_A ? PC_XA : (which == XK_B ? PC_XB : PC_UMAX), xk_bytes(m, a));
+  int rc = mixed(m) ? dispatch_xk<float>(which, m, a, t.d32) : dispatch_xk<double>(which, m, a, t.d64);
+  prof_end(m);
+  return rc;
 }
 
 static PhysArgs phys_base(Model& m) {
@@ -1018,9 +1088,17 @@ static int launch_k1(Model& m, int relax) {
   a.uold = m.uold;
   a.eta = m.eta; a.xi = m.xi;
   a.ctl = m.ctl;
+  {
+    const double nt = (double)m.nt, es = (double)m.esz;
+    double b = nt * (es + 16.0 + 8.0 + es);
+    if (m.evolve) b += nt * (3 * es + 3 * 16.0 * 2 + (relax ? 6 * 16.0 * 2 : 0.0));
+    prof_begin(m, PC_K1, b);
+  }
   if (mixed(m)) hipLaunchKernelGGL(k_spec1<float>, dim3(nblocks(m.nt)), dim3(256), 0, m.st, a);
   else hipLaunchKernelGGL(k_spec1<double>, dim3(nblocks(m.nt)), dim3(256), 0, m.st, a);
+  prof_end(m);
   HY_HIP(hipGetLastError());
+  HY_TRACE("spec_mu", m.st);
   return HY_OK;
 }
 
@@ -1041,9 +1119,20 @@ static int launch_k2(Model& m, int stage_idx, int rhs_only) {
   a.stage = stage_idx;
   a.rhs_only = rhs_only;
   a.ctl = m.ctl;
+  {
+    const double nt = (double)m.nt, es = (double)m.esz;
+    const int nd = m.g.ndim;
+    const int nfl = nd * (nd + 1) / 2;
+    const int ns = m.evolve ? 10 : 4;
+    double b = nt * es * (nfl + nd + (m.stress_newt ? nd : 0) + (m.evolve ? 6 : 0));
+    b += nt * 16.0 * (ns * 2 + (rhs_only ? ns : ns * 2));
+    prof_begin(m, PC_K2, b);
+  }
   if (mixed(m)) hipLaunchKernelGGL(k_spec2<float>, dim3(nblocks(m.nt)), dim3(256), 0, m.st, a);
   else hipLaunchKernelGGL(k_spec2<double>, dim3(nblocks(m.nt)), dim3(256), 0, m.st, a);
+  prof_end(m);
   HY_HIP(hipGetLastError());
+  HY_TRACE("spec_update", m.st);
   return HY_OK;
 }
 
@@ -1158,6 +1247,10 @@ static void model_free(Model& m) {
   for (void* p : {(void*)m.S, (void*)m.R, (void*)m.uold, (void*)m.kcomb, (void*)m.fext, (void*)m.fdtab,
                   m.xb, m.zb, m.fb, (void*)m.red, (void*)m.ctl, (void*)m.probe})
     if (p) (void)hipFree(p);
+  for (auto& e : m.evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
+  for (auto e : m.pool) (void)hipEventDestroy(e);
+  m.evs.clear();
+  m.pool.clear();
   if (m.st) (void)hipStreamDestroy(m.st);
   grid_free(m.g);
 }
@@ -1172,6 +1265,235 @@ static int dmalloc(void** p, size_t bytes) {
 }
 
 int model_build(Model& m, const hy_model_desc* d);
+int fft_r2c_full_api(Grid& g, const double* u, c64* uhat, hipStream_t s);
+int fft_c2r_full_api(Grid& g, const c64* uhat, double* u, hipStream_t s);
+
+// ------------------------------------------------- functional mu and budgets
+// Real-space evaluation in FP64 on the full half spectrum, following
+// functional_mu (functional.f90:56-67) and diag_budgets (diagnostics.f90:41-60).
+struct FullArgs {
+  int nkx, ny, nz, Kx, My, Mz;
+  const double *kx, *ky, *kz;
+};
+__device__ inline bool fkept(const FullArgs& a, int i, int j, int k) {
+  int my = j <= a.ny / 2 ? j : j - a.ny, mz = k <= a.nz / 2 ? k : k - a.nz;
+  return i < a.Kx && abs(my) <= a.My && abs(mz) <= a.Mz;
+}
+// out = mask (kh N + kr (N - n0 delta_k0))  | op 1: out = mask i k_dir N | op 2: out = -mask k^2 N
+__global__ void k_full_conv(FullArgs a, int op, int dir, const c64* __restrict__ N, const double* __restrict__ kh,
+                            const double* __restrict__ kr, double n0, c64* __restrict__ out) {
+  size_t n = (size_t)a.nkx * a.ny * a.nz;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    int i = (int)(e % a.nkx), j = (int)((e / a.nkx) % a.ny), k = (int)(e / ((size_t)a.nkx * a.ny));
+    double m = fkept(a, i, j, k) ? 1.0 : 0.0;
+    c64 v = N[e];
+    c64 r;
+    if (op == 0) {
+      c64 vp = v;
+      if (e == 0) vp.x -= n0;
+      r = scal(v, kh[e]) + scal(vp, kr[e]);
+    } else if (op == 1) {
+      double kd = dir == 0 ? a.kx[i] : (dir == 1 ? a.ky[j] : a.kz[k]);
+      r = mk<double>(-kd * v.y, kd * v.x);
+    } else {
+      double k2 = a.kx[i] * a.kx[i] + a.ky[j] * a.ky[j] + a.kz[k] * a.kz[k];
+      r = scal(v, -k2);
+    }
+    out[e] = scal(r, m);
+  }
+}
+
+struct PointArgs {
+  size_t nr;
+  int nx, ny, nz, nd;
+  double dx, dy, dz, lx, ly, lz;
+  const double* n;
+  const double* mom;     // 3 fields (budgets)
+  const double* conv;    // mu: convolution sum | budgets: hartree w
+  const double* wry;     // budgets: ry w
+  const double* gn;      // 3 gradient fields (gradient term)
+  const double* lap;
+  double n0, beta, cn, alpha0, cg;
+  int ideal, tf, grad, hartree, ry;
+  const double* fdtab;
+  // external potential for the budgets
+  int vext_kind;
+  double vamp, vt, vomega, vramp, vwidth, kvec[3];
+  int vext_td;
+  double* out;           // mu output | budgets: per-block partials (8 per block)
+  int* flag;             // set on non-positive density
+};
+
+__global__ void k_point_mu(PointArgs a) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < a.nr; e += (size_t)gridDim.x * blockDim.x) {
+    double n = a.n[e];
+    if (!(n > 0.0)) { *a.flag = 1; a.out[e] = 0.0; continue; }
+    double mu = a.conv[e];
+    if (a.ideal) mu += log(n) / a.beta;
+    if (a.tf || a.grad) {
+      double al = fd::alpha_from_ip12(a.fdtab, n / a.cn);
+      if (a.tf) mu += al / a.beta;
+      if (a.grad) {
+        double xi = fd::xi(a.fdtab, al);
+        double dxidn = fd::dxi(a.fdtab, al) / (0.5 * a.cn * fd::im12(a.fdtab, al));
+        double g2 = a.gn[e] * a.gn[e] + a.gn[a.nr + e] * a.gn[a.nr + e] + a.gn[2 * a.nr + e] * a.gn[2 * a.nr + e];
+        mu -= a.cg * (dxidn * g2 + 2.0 * xi * a.lap[e]);
+      }
+    }
+    a.out[e] = mu;
+  }
+}
+
+__device__ inline double vext_at(const PointArgs& a, size_t e) {
+  int i = (int)(e % a.nx), j = (int)((e / a.nx) % a.ny), k = (int)(e / ((size_t)a.nx * a.ny));
+  double x = i * a.dx, y = j * a.dy, z = k * a.dz;
+  if (a.vext_kind == HY_VEXT_MODE || a.vext_kind == HY_VEXT_DRIVEN) {
+    double t = a.vext_td ? a.vt : 0.0;
+    double amp = a.vamp;
+    if (a.vext_td && a.vramp > 0.0) amp = amp * fmin(1.0, t / a.vramp);
+    double ph = a.kvec[0] * x + a.kvec[1] * y + a.kvec[2] * z - a.vomega * t;
+    return amp * cos(ph);
+  }
+  if (a.vext_kind == HY_VEXT_GAUSSIAN) {
+    double xx = x - 0.5 * a.lx, yy = a.ny > 1 ? y - 0.5 * a.ly : 0.0, zz = a.nz > 1 ? z - 0.5 * a.lz : 0.0;
+    double r2 = xx * xx + yy * yy + zz * zz;
+    return -a.vamp * exp(-0.5 * r2 / (a.vwidth * a.vwidth));
+  }
+  return 0.0;
+}
+
+// partial sums: 0 sum mom^2/n, 1 n(ln n - 1), 2 n w_h, 3 (n - n0) w_ry,
+// 4 n alpha - 2/3 cn I_3/2(alpha), 5 xi |grad n|^2, 6 n v_ext
+__global__ void k_point_budget(PointArgs a) {
+  __shared__ double sh[7][256];
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < a.nr; e += (size_t)gridDim.x * blockDim.x) {
+    double n = a.n[e];
+    if (!(n > 0.0)) { *a.flag = 1; continue; }
+    for (int d = 0; d < a.nd; ++d) acc[0] += a.mom[d * a.nr + e] * a.mom[d * a.nr + e] / n;
+    if (a.ideal) acc[1] += n * (log(n) - 1.0);
+    if (a.hartree) acc[2] += n * a.conv[e];
+    if (a.ry) acc[3] += (n - a.n0) * a.wry[e];
+    if (a.tf || a.grad) {
+      double al = fd::alpha_from_ip12(a.fdtab, n / a.cn);
+      if (a.tf) acc[4] += n * al - (2.0 / 3.0) * a.cn * fd::ip32(a.fdtab, al);
+      if (a.grad) {
+        double g2 = a.gn[e] * a.gn[e] + a.gn[a.nr + e] * a.gn[a.nr + e] + a.gn[2 * a.nr + e] * a.gn[2 * a.nr + e];
+        acc[5] += fd::xi(a.fdtab, al) * g2;
+      }
+    }
+    if (a.vext_kind != HY_VEXT_NONE) acc[6] += n * vext_at(a, e);
+  }
+  for (int q = 0; q < 7; ++q) sh[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int q = 0; q < 7; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q < 7; ++q) a.out[blockIdx.x * 8 + q] = sh[q][0];
+}
+
+static FullArgs full_args_m(const Model& m) {
+  FullArgs a;
+  const Grid& g = m.g;
+  a.nkx = g.nkx; a.ny = g.ny; a.nz = g.nz; a.Kx = g.Kx; a.My = g.My; a.Mz = g.Mz;
+  a.kx = g.d_kx; a.ky = g.d_ky; a.kz = g.d_kz;
+  return a;
+}
+
+// hartree v(k) and ry -c_cor(k)/beta on the full half spectrum
+static int full_kernels(Model& m, double** kh, double** kr) {
+  const Grid& g = m.g;
+  std::vector<double> h(g.nfull(), 0.0), r(g.nfull(), 0.0);
+  for (int k = 0; k < g.nz; ++k)
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nkx; ++i) {
+        size_t e = ((size_t)k * g.ny + j) * g.nkx + i;
+        double kabs = std::sqrt(g.kx[i] * g.kx[i] + g.ky[j] * g.ky[j] + g.kz[k] * g.kz[k]);
+        for (size_t t = 0; t < m.terms.size(); ++t) {
+          if (m.terms[t].kind == HY_TERM_HARTREE) h[e] += term_kernel(m, t, kabs);
+          if (m.terms[t].kind == HY_TERM_RY) r[e] += term_kernel(m, t, kabs);
+        }
+      }
+  HY_TRY(dmalloc((void**)kh, sizeof(double) * g.nfull()));
+  HY_TRY(dmalloc((void**)kr, sizeof(double) * g.nfull()));
+  HY_HIP(hipMemcpy(*kh, h.data(), sizeof(double) * g.nfull(), hipMemcpyHostToDevice));
+  HY_HIP(hipMemcpy(*kr, r.data(), sizeof(double) * g.nfull(), hipMemcpyHostToDevice));
+  return HY_OK;
+}
+
+struct DevBufs {
+  std::vector<void*> p;
+  ~DevBufs() { for (void* q : p) if (q) (void)hipFree(q); }
+  template <class T> int get(T** out, size_t n) {
+    void* q = nullptr;
+    HY_TRY(dmalloc(&q, sizeof(T) * std::max<size_t>(n, 1)));
+    p.push_back(q);
+    *out = (T*)q;
+    return HY_OK;
+  }
+};
+
+// real fields needed by mu and budgets, from a real density on the device:
+// Nhat, conv (hartree+ry or hartree only), wry, grad n, lap n
+static int real_terms(Model& m, DevBufs& B, const double* dn, bool split, c64** Nh, double** conv,
+                      double** wry, double** gn, double** lap) {
+  Grid& g = m.g;
+  const size_t nr = g.nreal(), nf = g.nfull();
+  c64 *N, *W;
+  HY_TRY(B.get(&N, nf));
+  HY_TRY(B.get(&W, nf));
+  double *kh, *kr;
+  HY_TRY(full_kernels(m, &kh, &kr));
+  B.p.push_back(kh);
+  B.p.push_back(kr);
+  HY_TRY(fft_r2c_full_api(g, dn, N, m.st));
+  FullArgs fa = full_args_m(m);
+  const int nb = nblocks(nf);
+  HY_TRY(B.get(conv, nr));
+  double* zero;
+  HY_TRY(B.get(&zero, nf));
+  HY_HIP(hipMemsetAsync(zero, 0, sizeof(double) * nf, m.st));
+  hipLaunchKernelGGL(k_full_conv, dim3(nb), dim3(256), 0, m.st, fa, 0, 0, N, kh, split ? zero : kr, m.n0, W);
+  HY_TRY(fft_c2r_full_api(g, W, *conv, m.st));
+  *wry = nullptr;
+  if (split) {
+    HY_TRY(B.get(wry, nr));
+    hipLaunchKernelGGL(k_full_conv, dim3(nb), dim3(256), 0, m.st, fa, 0, 0, N, zero, kr, m.n0, W);
+    HY_TRY(fft_c2r_full_api(g, W, *wry, m.st));
+  }
+  *gn = nullptr; *lap = nullptr;
+  if (m.grad) {
+    HY_TRY(B.get(gn, nr * 3));
+    HY_TRY(B.get(lap, nr));
+    HY_HIP(hipMemsetAsync(*gn, 0, sizeof(double) * nr * 3, m.st));
+    for (int d = 0; d < g.ndim; ++d) {
+      hipLaunchKernelGGL(k_full_conv, dim3(nb), dim3(256), 0, m.st, fa, 1, d, N, kh, kr, m.n0, W);
+      HY_TRY(fft_c2r_full_api(g, W, *gn + nr * d, m.st));
+    }
+    hipLaunchKernelGGL(k_full_conv, dim3(nb), dim3(256), 0, m.st, fa, 2, 0, N, kh, kr, m.n0, W);
+    HY_TRY(fft_c2r_full_api(g, W, *lap, m.st));
+  }
+  HY_HIP(hipGetLastError());
+  *Nh = N;
+  return HY_OK;
+}
+
+static PointArgs point_args(Model& m) {
+  PointArgs a;
+  std::memset(&a, 0, sizeof(a));
+  const Grid& g = m.g;
+  a.nr = g.nreal(); a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nd = g.ndim;
+  a.dx = g.dx; a.dy = g.dy; a.dz = g.dz; a.lx = g.lx; a.ly = g.ly; a.lz = g.lz;
+  a.n0 = m.n0; a.beta = m.beta; a.cn = m.cn; a.alpha0 = m.alpha0; a.cg = m.cg;
+  a.ideal = m.ideal; a.tf = m.tf; a.grad = m.grad; a.hartree = m.hartree; a.ry = m.ry;
+  a.fdtab = m.fdtab;
+  a.vext_kind = m.vext_kind; a.vamp = m.vamp; a.vomega = m.vomega; a.vramp = m.vramp; a.vwidth = m.vwidth;
+  a.kvec[0] = m.kvec[0]; a.kvec[1] = m.kvec[1]; a.kvec[2] = m.kvec[2]; a.vext_td = m.vext_td;
+  return a;
+}
 
 }  // namespace hy
 
@@ -1489,18 +1811,22 @@ int hy_model_advance(hy_model h, int nsteps, int dt_every, double tend, const in
       m.probe_cap = nsteps;
     }
   }
+  // tend > 0: stop at tend and clamp the last step (src/hydft.f90:41,44);
+  // tend < 0: stop once t >= |tend| without clamping (the test loops).
+  const double tstop = std::fabs(tend);
+  const double tclamp = tend > 0.0 ? tend : 0.0;
   int taken = 0;
   for (int i = 0; i < nsteps; ++i) {
-    if (tend > 0.0) {
+    if (tend != 0.0) {
       Ctl c;
       HY_TRY(read_ctl(m, c));
-      if (c.t >= tend) break;
+      if (c.t >= tstop) break;
     }
     if (dt_every > 0) {
       if (m.irecompute == 0) {
         HY_TRY(finish_relax(m));
         HY_TRY(compute_dt_dev(m));
-        if (tend > 0.0 || i == 0) {
+        if (tend != 0.0 || i == 0) {
           Ctl c;
           HY_TRY(read_ctl(m, c));
           if (c.nodt) return set_error(HY_ERR_NO_DT, "cannot determine a time step; set dt in &time");
@@ -1508,7 +1834,7 @@ int hy_model_advance(hy_model h, int nsteps, int dt_every, double tend, const in
       }
       m.irecompute = (m.irecompute + 1) % dt_every;
     }
-    HY_TRY(step_once(m, tend));
+    HY_TRY(step_once(m, tclamp));
     if (probe3) hipLaunchKernelGGL(k_probe, dim3(1), dim3(1), 0, m.st, m.ctl, m.S, m.nt, pidx, m.n0, m.probe + 6 * (size_t)taken);
     ++taken;
   }
@@ -1564,6 +1890,124 @@ int hy_model_probe(hy_model h, int mx, int my, int mz, double* out4) {
   out4[0] = a.x / m.n0; out4[1] = a.y / m.n0; out4[2] = b.x; out4[3] = b.y;
   return HY_OK;
 }
+
+
+int hy_model_mu(hy_model h, const double* h_n, double* h_mu) {
+  if (!h || !h_n || !h_mu) return set_error(HY_ERR_INVALID, "null argument");
+  Model& m = h->m;
+  const size_t nr = m.g.nreal();
+  DevBufs B;
+  double *dn, *out;
+  int* flag;
+  HY_TRY(B.get(&dn, nr));
+  HY_TRY(B.get(&out, nr));
+  HY_TRY(B.get(&flag, 1));
+  HY_HIP(hipMemsetAsync(flag, 0, sizeof(int), m.st));
+  HY_HIP(hipMemcpyAsync(dn, h_n, sizeof(double) * nr, hipMemcpyHostToDevice, m.st));
+  c64* N;
+  double *conv, *wry, *gn, *lap;
+  HY_TRY(real_terms(m, B, dn, false, &N, &conv, &wry, &gn, &lap));
+  PointArgs a = point_args(m);
+  a.n = dn; a.conv = conv; a.gn = gn; a.lap = lap; a.out = out; a.flag = flag;
+  hipLaunchKernelGGL(k_point_mu, dim3(nblocks(nr)), dim3(256), 0, m.st, a);
+  HY_HIP(hipGetLastError());
+  int bad = 0;
+  HY_HIP(hipMemcpyAsync(h_mu, out, sizeof(double) * nr, hipMemcpyDeviceToHost, m.st));
+  HY_HIP(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, m.st));
+  HY_HIP(hipStreamSynchronize(m.st));
+  if (bad) return set_error(HY_ERR_NONPOSITIVE, "density is not positive");
+  return HY_OK;
+}
+
+int hy_model_budgets(hy_model h, double* out6) {
+  if (!h || !out6) return set_error(HY_ERR_INVALID, "null argument");
+  Model& m = h->m;
+  const Grid& g = m.g;
+  HY_TRY(finish_relax(m));
+  const size_t nr = g.nreal();
+  DevBufs B;
+  double *dn, *dmom, *dn0;
+  c64* F;
+  int* flag;
+  HY_TRY(B.get(&dn, nr));
+  HY_TRY(B.get(&dmom, nr * 3));
+  HY_TRY(B.get(&F, g.nfull()));
+  HY_TRY(B.get(&flag, 1));
+  HY_HIP(hipMemsetAsync(flag, 0, sizeof(int), m.st));
+  SpecArgs sa = spec_args(m);
+  for (int f = 0; f < 4; ++f) {
+    hipLaunchKernelGGL(k_trunc_to_full, dim3(nblocks(g.nfull())), dim3(256), 0, m.st, sfield(m, f), F, sa, g.nkx);
+    HY_TRY(fft_c2r_full_api(m.g, F, f == 0 ? dn : dmom + nr * (f - 1), m.st));
+  }
+  (void)dn0;
+  c64* N;
+  double *conv, *wry, *gn, *lap;
+  HY_TRY(real_terms(m, B, dn, true, &N, &conv, &wry, &gn, &lap));
+  Ctl c;
+  HY_TRY(read_ctl(m, c));
+  PointArgs a = point_args(m);
+  a.n = dn; a.mom = dmom; a.conv = conv; a.wry = wry; a.gn = gn; a.lap = lap; a.flag = flag;
+  a.vt = c.step > 0 ? c.t_vext : 0.0;
+  const int nbk = 256;
+  double* part;
+  HY_TRY(B.get(&part, nbk * 8));
+  a.out = part;
+  hipLaunchKernelGGL(k_point_budget, dim3(nbk), dim3(256), 0, m.st, a);
+  HY_HIP(hipGetLastError());
+  std::vector<double> hp(nbk * 8);
+  c64 k0[4];
+  int bad = 0;
+  HY_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * nbk * 8, hipMemcpyDeviceToHost, m.st));
+  for (int f = 0; f < 4; ++f) HY_HIP(hipMemcpyAsync(&k0[f], sfield(m, f), sizeof(c64), hipMemcpyDeviceToHost, m.st));
+  HY_HIP(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, m.st));
+  HY_HIP(hipStreamSynchronize(m.st));
+  if (bad) return set_error(HY_ERR_NONPOSITIVE, "density is not positive");
+  double sum[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int b = 0; b < nbk; ++b)
+    for (int q = 0; q < 7; ++q) sum[q] += hp[b * 8 + q];
+  const double V = g.volume, Nr = (double)nr;
+  // mass and momentum are the k = 0 modes of the spectral state (mean(u) = u_hat(0))
+  out6[0] = k0[0].x * V;
+  for (int d = 0; d < 3; ++d) out6[1 + d] = k0[1 + d].x * V;
+  out6[4] = 0.5 * (sum[0] / Nr) * V / m.mass;
+  double fe = 0.0;
+  if (m.ideal) fe += (sum[1] / Nr) * V / m.beta;
+  if (m.hartree) fe += 0.5 * (sum[2] / Nr) * V;
+  if (m.ry) fe += -0.5 * (sum[3] / Nr) * V / m.beta;
+  if (m.tf) fe += (sum[4] / Nr) * V / m.beta;
+  if (m.grad) fe += m.cg * (sum[5] / Nr) * V;
+  if (m.vext_kind != HY_VEXT_NONE) fe += (sum[6] / Nr) * V;
+  out6[5] = fe;
+  return HY_OK;
+}
+
+int hy_model_profile(hy_model h, int enable) {
+  if (!h) return set_error(HY_ERR_INVALID, "null argument");
+  Model& m = h->m;
+  HY_HIP(hipStreamSynchronize(m.st));
+  for (auto& e : m.evs) { m.pool.push_back(e.a); m.pool.push_back(e.b); }
+  m.evs.clear();
+  m.prof = enable != 0;
+  return HY_OK;
+}
+
+int hy_model_profile_read(hy_model h, int maxcls, int* launches, double* total_ms, double* bytes) {
+  if (!h || !launches || !total_ms || !bytes) return set_error(HY_ERR_INVALID, "null argument");
+  Model& m = h->m;
+  HY_HIP(hipStreamSynchronize(m.st));
+  for (int c = 0; c < maxcls; ++c) { launches[c] = 0; total_ms[c] = 0.0; bytes[c] = 0.0; }
+  for (auto& e : m.evs) {
+    if (e.cls >= maxcls) continue;
+    float ms = 0.0f;
+    HY_HIP(hipEventElapsedTime(&ms, e.a, e.b));
+    launches[e.cls] += 1;
+    total_ms[e.cls] += ms;
+    bytes[e.cls] += e.bytes;
+  }
+  return PC_N;
+}
+
+const char* hy_profile_class_name(int cls) { return (cls >= 0 && cls < PC_N) ? prof_names[cls] : ""; }
 
 int hy_model_kernel(hy_model h, double k, double* out) {
   if (!h || !out) return set_error(HY_ERR_INVALID, "null argument");

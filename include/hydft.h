@@ -140,7 +140,8 @@ int hy_model_compute_dt(hy_model m, double* h_dt);
 /* model_advance x nsteps (model.f90:186-227).  dt_every = 0 keeps the current
    dt; k > 0 calls compute_dt before steps 0, k, 2k, ... of this call (the
    driver cadence of src/hydft.f90:42-43 with k = 20).  tend > 0 clamps the
-   last step to tend and stops there (src/hydft.f90:41,44).  When probe3 is not
+   last step to tend and stops there (src/hydft.f90:41,44); tend < 0 stops
+   once t >= -tend without clamping (the loops of the reference's tests).  When probe3 is not
    NULL, mode probe3 of n/n0 and mom_x (diagnostics.f90:70-77) is recorded
    after every step into h_probe (6 doubles per step: t, Re, Im dn_k/n0,
    Re, Im mom_k, dt).  Returns the number of steps taken in *h_taken. */
@@ -156,6 +157,16 @@ int hy_model_budgets(hy_model m, double* h_out6);
 int hy_model_probe(hy_model m, int mx, int my, int mz, double* h_out4);
 /* functional_kernel K(k) (functional.f90:64-72) */
 int hy_model_kernel(hy_model m, double k, double* h_K);
+
+/* ---- measurement (no counterpart in the reference) ------------------------
+   hy_model_profile(m, 1) clears and starts recording a HIP event pair around
+   every kernel launch of the model, on the model's stream; _read returns, per
+   kernel class, the launch count, the summed event time (ms) and the summed
+   algorithmic bytes (compulsory reads + writes of each launch), and returns
+   the number of classes.  Class names come from hy_profile_class_name. */
+int hy_model_profile(hy_model m, int enable);
+int hy_model_profile_read(hy_model m, int maxcls, int* launches, double* total_ms, double* bytes);
+const char* hy_profile_class_name(int cls);
 
 #ifdef __cplusplus
 }

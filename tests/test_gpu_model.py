@@ -89,3 +89,26 @@ def test_steps_match_oracle(gpu, name):
                 assert P.rel_l2(pi[c], om.pi[c]) <= 1e-9, (name, c)
     t, dt, step = gm.time
     assert step == steps and abs(t - om.t) <= 1e-14 * max(1.0, om.t)
+
+
+@pytest.mark.parametrize("name", ["ion3d_newt_visc", "elec3d", "elec1d", "ion2d_gaussian", "ion1d_driven"])
+def test_mu_and_budgets_match_oracle(gpu, name):
+    p, gm, om, steps = build(name)
+    # functional_mu on a non-band-limited positive density
+    rng = np.random.default_rng(4)
+    n = P.N0 * (1.0 + 0.05 * (rng.random(om.spc.g.shape) - 0.5))
+    mu_g = gm.mu(n)
+    mu_o = om.fun.mu(om.spc, n, om.spc.fwd(n))
+    assert np.abs(mu_g - mu_o).max() <= 1e-11 * np.abs(mu_o).max()
+    # diag_budgets after a few steps (the oracle's v_ext is re-evaluated at its last stage time)
+    om.dt = p.dt
+    for _ in range(steps):
+        om.advance()
+    gm.advance(steps)
+    mass, mom, ekin, fe = gm.budgets()
+    omass, omom, oekin, ofe = om.budgets()
+    assert abs(mass - omass) <= 1e-13 * abs(omass)
+    for d in range(3):
+        assert abs(mom[d] - omom[d]) <= 1e-12 * abs(omass) * 1e-2 + 1e-10 * abs(omom[d])
+    assert abs(ekin - oekin) <= 1e-9 * abs(oekin)
+    assert abs(fe - ofe) <= 1e-11 * abs(ofe)
