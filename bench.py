@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""RK3 steps/s of the 3-D pseudo-spectral DDFT-QHD model on one MI355X.
+
+Workload (default): quantum electrons, Thomas-Fermi + Kirzhnits gradient +
+Coulomb Hartree + RY correlation from the Hansen-McDonald HNC c_ee, Newtonian
+electron_fit viscosity, r_s = 1, theta = 1, 256^3 periodic grid in a box
+L = 2 pi/0.05; FP32 spectral kernels with an FP64 RK3 state.  Initial dn:
+seeded random phases on the shell 0.1 < q < 1.5 with rms 5e-3 n0; u: three
+independent seeded fields on the same shell with 1e-3 rms each, mom = n u.
+dt is the CFL step (cfl = 0.4) at t = 0, frozen.  Synthetic, seeded inputs.
+
+One JSON line on stdout.  `roofline` is for the kernel class with the largest
+summed time in the timed region (HIP events on the model's stream; algorithmic
+bytes = compulsory reads + writes of each launch).  `cpu_baseline` times one
+RK3 stage of the CPU FP64 oracle (the reference dataflow, single core) on the
+same workload and scales it to steps/s.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def config(n=256, precision="mixed"):
+    from hydft_amd.host.params import Params
+    p = Params()
+    p.kind, p.rs, p.theta = "electron", 1.0, 1.0
+    p.nx = p.ny = p.nz = n
+    p.lx = p.ly = p.lz = 2.0 * math.pi / 0.05
+    p.terms = "tf,gradient,hartree,ry"
+    p.transport, p.closure = "electron_fit", "newtonian"
+    p.cfl = 0.4
+    p.precision = precision
+    return p.finalize()
+
+
+def initial_state(p, n0, seed=1234):
+    from hydft_amd.host.initial import shell_rms_field
+    dn = shell_rms_field(p, 5e-3 * n0, 0.1, 1.5, seed)
+    n = n0 + dn
+    u = np.stack([shell_rms_field(p, 1e-3, 0.1, 1.5, seed + 1 + d) for d in range(3)])
+    return n, n[None] * u
+
+
+def profile_table(m, ncls=10):
+    from hydft_amd import abi
+    launches = (C.c_int * ncls)()
+    ms = (C.c_double * ncls)()
+    by = (C.c_double * ncls)()
+    n = abi.lib().lib.hy_model_profile_read(m.h, ncls, launches, ms, by)
+    out = []
+    for c in range(min(n, ncls)):
+        if launches[c]:
+            out.append({"kernel": abi.lib().lib.hy_profile_class_name(c).decode(), "launches": launches[c],
+                        "total_ms": ms[c], "bytes": by[c]})
+    return out
+
+
+def cpu_baseline(p, st, n, mom):
+    """One RK3 stage of the oracle (reference dataflow, FP64, one core)."""
+    from oracle import hydft_oracle as O
+    om = O.make_model(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, kind=p.kind, gamma=p.gamma, rs=p.rs,
+                      terms=p.terms, k_tab=st.k, c_tab=st.c, eta=0.0, dt=1e-3)
+    om.n = n.copy()
+    om.mom = [mom[d].copy() for d in range(3)]
+    t0 = time.perf_counter()
+    om.advance_stage(0)
+    el = time.perf_counter() - t0
+    return {"value": 1.0 / (3.0 * el), "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"one RK3 stage (of 3) of the FP64 oracle at {p.nx}^3, {el:.1f} s, scaled to steps/s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; runs on more than one GPU are not part of it "
+              "(independent runs would shard with no exchange).", file=sys.stderr)
+        return 1
+
+    import torch
+    from hydft_amd import abi
+    from hydft_amd.model import Model, build_species
+    from hydft_amd.host import hnc
+
+    abi.call("hy_init", 0)
+    p = config(args.n, args.precision)
+    t_setup = time.perf_counter()
+    un_sp = build_species(p)  # HNC (Hansen-McDonald), electron_fit transport
+    st = un_sp.structure
+    m = Model(p, species=un_sp, precision=args.precision)
+    n, mom = initial_state(p, m.n0)
+    m.set_state(n, mom)
+    dt = m.compute_dt()
+    m.set_dt(dt)
+    m.p.dt = dt
+    print(f"# setup {time.perf_counter() - t_setup:.1f} s, dt = {dt:.4e}, n0 = {m.n0:.6f}", file=sys.stderr)
+    m.advance(args.warmup)
+    torch.cuda.synchronize()
+    abi.call("hy_model_profile", m.h, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m.advance(args.steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    prof = profile_table(m)
+    abi.call("hy_model_profile", m.h, 0)
+    sps = args.steps / el
+    dom = max(prof, key=lambda r: r["total_ms"])
+    avg_ms = dom["total_ms"] / dom["launches"]
+    bpl = dom["bytes"] / dom["launches"]
+    achieved = bpl / (avg_ms * 1e-3) / 1e9
+    tot_bytes = sum(r["bytes"] for r in prof)
+    tot_ms = sum(r["total_ms"] for r in prof)
+    extra = {"kernels": [{**r, "avg_us": 1e3 * r["total_ms"] / r["launches"],
+                          "GBps": r["bytes"] / (r["total_ms"] * 1e-3) / 1e9} for r in prof],
+             "step_hbm_GBps": tot_bytes / (tot_ms * 1e-3) / 1e9,
+             "bytes_per_step": tot_bytes / args.steps}
+    # FP64 comparison on the same input: speed and the density error of the mixed run
+    if not args.no_fp64 and args.precision == "mixed":
+        nm, _, _ = m.get_state()
+        m64 = Model(p, species=un_sp, precision="fp64")
+        m64.set_state(n, mom)
+        m64.set_dt(dt)
+        m64.advance(args.warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m64.advance(args.steps)
+        torch.cuda.synchronize()
+        el64 = time.perf_counter() - t0
+        n64, _, _ = m64.get_state()
+        extra["fp64_steps_per_s"] = args.steps / el64
+        extra["mixed_over_fp64"] = sps / extra["fp64_steps_per_s"]
+        extra["mixed_density_rel_l2_vs_fp64"] = float(np.linalg.norm(nm - n64) / np.linalg.norm(n64))
+        extra["mixed_dn_rel_l2_vs_fp64"] = float(np.linalg.norm(nm - n64) / np.linalg.norm(n64 - m.n0))
+        extra["steps_compared"] = args.warmup + args.steps
+        m64.close()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(p, st, n, mom)
+    out = {
+        "metric": "RK3 steps/s at 256^3 (DDFT-QHD electrons, FP32-mixed)" if args.n == 256 else
+                  f"RK3 steps/s at {args.n}^3 (DDFT-QHD electrons)",
+        "value": sps,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 / sps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 transforms, f64 state" if args.precision == "mixed" else "f64",
+        "data": "synthetic (seeded shell-random dn and u)",
+        "config": {"workload": f"electrons_ddft_qhd_{args.n}^3", "grid": [args.n] * 3, "box": p.lx,
+                   "rs": p.rs, "theta": p.theta, "terms": p.terms, "closure": p.closure,
+                   "transport": p.transport, "precision": args.precision, "dt": dt},
+        "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "bytes_per_launch": bpl, "avg_launch_us": avg_ms * 1e3},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

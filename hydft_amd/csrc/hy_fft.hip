@@ -438,8 +438,10 @@ __device__ inline bool full_kept(const FullSpecArgs& a, int i, int j, int k) {
   return i < a.Kx && (my < 0 ? -my : my) <= a.My && (mz < 0 ? -mz : mz) <= a.Mz;
 }
 
-__global__ void k_specop(FullSpecArgs a, int op, int dir, const c64* __restrict__ in,
-                         const double* __restrict__ kern, c64* __restrict__ out) {
+// kdir: wavenumbers of the operator's axis, indexed by (e / kstride) % kn; the
+// host selects the array (no pointer selection inside the kernel).
+__global__ void k_specop(FullSpecArgs a, int op, const double* __restrict__ kdir, size_t kstride, int kn,
+                         const c64* __restrict__ in, const double* __restrict__ kern, c64* __restrict__ out) {
   size_t n = (size_t)a.nkx * a.ny * a.nz;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     int i = e % a.nkx;
@@ -447,7 +449,7 @@ __global__ void k_specop(FullSpecArgs a, int op, int dir, const c64* __restrict_
     int k = e / ((size_t)a.nkx * a.ny);
     double m = full_kept(a, i, j, k) ? 1.0 : 0.0;
     c64 v = in[e];
-    double kd = dir == 0 ? a.kx[i] : (dir == 1 ? a.ky[j] : a.kz[k]);
+    double kd = kdir[(e / kstride) % (size_t)kn];
     switch (op) {
       case SOP_COPY: out[e] = v; break;
       case SOP_MASK: out[e] = scal(v, m); break;
@@ -473,7 +475,10 @@ static int specop(const Grid& g, int op, int dir, const c64* in, const double* k
                   hipStream_t s) {
   size_t n = g.nfull();
   int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_specop, dim3(blocks), dim3(256), 0, s, full_args(g), op, dir, in, kern, out);
+  const double* kdir = dir == 0 ? g.d_kx : (dir == 1 ? g.d_ky : g.d_kz);
+  size_t kstride = dir == 0 ? 1 : (dir == 1 ? (size_t)g.nkx : (size_t)g.nkx * g.ny);
+  int kn = dir == 0 ? g.nkx : (dir == 1 ? g.ny : g.nz);
+  hipLaunchKernelGGL(k_specop, dim3(blocks), dim3(256), 0, s, full_args(g), op, kdir, kstride, kn, in, kern, out);
   HY_HIP(hipGetLastError());
   HY_TRACE("specop", s);
   return HY_OK;

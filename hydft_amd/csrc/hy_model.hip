@@ -1280,7 +1280,8 @@ __device__ inline bool fkept(const FullArgs& a, int i, int j, int k) {
   return i < a.Kx && abs(my) <= a.My && abs(mz) <= a.Mz;
 }
 // out = mask (kh N + kr (N - n0 delta_k0))  | op 1: out = mask i k_dir N | op 2: out = -mask k^2 N
-__global__ void k_full_conv(FullArgs a, int op, int dir, const c64* __restrict__ N, const double* __restrict__ kh,
+__global__ void k_full_conv(FullArgs a, int op, const double* __restrict__ kdir, size_t kstride, int kn,
+                            const c64* __restrict__ N, const double* __restrict__ kh,
                             const double* __restrict__ kr, double n0, c64* __restrict__ out) {
   size_t n = (size_t)a.nkx * a.ny * a.nz;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
@@ -1293,7 +1294,7 @@ __global__ void k_full_conv(FullArgs a, int op, int dir, const c64* __restrict__
       if (e == 0) vp.x -= n0;
       r = scal(v, kh[e]) + scal(vp, kr[e]);
     } else if (op == 1) {
-      double kd = dir == 0 ? a.kx[i] : (dir == 1 ? a.ky[j] : a.kz[k]);
+      double kd = kdir[(e / kstride) % (size_t)kn];
       r = mk<double>(-kd * v.y, kd * v.x);
     } else {
       double k2 = a.kx[i] * a.kx[i] + a.ky[j] * a.ky[j] + a.kz[k] * a.kz[k];
@@ -1456,12 +1457,14 @@ static int real_terms(Model& m, DevBufs& B, const double* dn, bool split, c64** 
   double* zero;
   HY_TRY(B.get(&zero, nf));
   HY_HIP(hipMemsetAsync(zero, 0, sizeof(double) * nf, m.st));
-  hipLaunchKernelGGL(k_full_conv, dim3(nb), dim3(256), 0, m.st, fa, 0, 0, N, kh, split ? zero : kr, m.n0, W);
+  hipLaunchKernelGGL(k_full_conv, dim3(nb), dim3(256), 0, m.st, fa, 0, g.d_kx, (size_t)1, g.nkx, N, kh,
+                     split ? zero : kr, m.n0, W);
   HY_TRY(fft_c2r_full_api(g, W, *conv, m.st));
   *wry = nullptr;
   if (split) {
     HY_TRY(B.get(wry, nr));
-    hipLaunchKernelGGL(k_full_conv, dim3(nb), dim3(256), 0, m.st, fa, 0, 0, N, zero, kr, m.n0, W);
+    hipLaunchKernelGGL(k_full_conv, dim3(nb), dim3(256), 0, m.st, fa, 0, g.d_kx, (size_t)1, g.nkx, N, zero, kr,
+                       m.n0, W);
     HY_TRY(fft_c2r_full_api(g, W, *wry, m.st));
   }
   *gn = nullptr; *lap = nullptr;
@@ -1470,10 +1473,14 @@ static int real_terms(Model& m, DevBufs& B, const double* dn, bool split, c64** 
     HY_TRY(B.get(lap, nr));
     HY_HIP(hipMemsetAsync(*gn, 0, sizeof(double) * nr * 3, m.st));
     for (int d = 0; d < g.ndim; ++d) {
-      hipLaunchKernelGGL(k_full_conv, dim3(nb), dim3(256), 0, m.st, fa, 1, d, N, kh, kr, m.n0, W);
+      const double* kdir = d == 0 ? g.d_kx : (d == 1 ? g.d_ky : g.d_kz);
+      size_t kstride = d == 0 ? 1 : (d == 1 ? (size_t)g.nkx : (size_t)g.nkx * g.ny);
+      int kn = d == 0 ? g.nkx : (d == 1 ? g.ny : g.nz);
+      hipLaunchKernelGGL(k_full_conv, dim3(nb), dim3(256), 0, m.st, fa, 1, kdir, kstride, kn, N, kh, kr, m.n0, W);
       HY_TRY(fft_c2r_full_api(g, W, *gn + nr * d, m.st));
     }
-    hipLaunchKernelGGL(k_full_conv, dim3(nb), dim3(256), 0, m.st, fa, 2, 0, N, kh, kr, m.n0, W);
+    hipLaunchKernelGGL(k_full_conv, dim3(nb), dim3(256), 0, m.st, fa, 2, g.d_kx, (size_t)1, g.nkx, N, kh, kr,
+                       m.n0, W);
     HY_TRY(fft_c2r_full_api(g, W, *lap, m.st));
   }
   HY_HIP(hipGetLastError());

@@ -873,43 +873,50 @@ class Model:
 
     def advance(self):
         """model.f90:186-227."""
-        spc, g = self.spc, self.spc.g
-        ts = self.t
-        dt = self.dt
+        self._ts = self.t
         for s in range(3):
-            dn, dmom, dpi = self.rhs(ts)
-            if self.stress.evolve:
-                pi0_old = self.stress.pi0(spc, self.u)
-            if s == 0:
-                self.n = self.n + dt * RK_A[0] * dn
-                self.mom = [self.mom[i] + dt * RK_A[0] * dmom[i] for i in range(3)]
-                if self.stress.evolve:
-                    self.pi = [self.pi[c] + dt * RK_A[0] * dpi[c] for c in range(NCOMP)]
-            else:
-                self.n = self.n + dt * (RK_A[s] * dn + RK_B[s] * self.dn_o)
-                self.mom = [self.mom[i] + dt * (RK_A[s] * dmom[i] + RK_B[s] * self.dmom_o[i]) for i in range(3)]
-                if self.stress.evolve:
-                    self.pi = [self.pi[c] + dt * (RK_A[s] * dpi[c] + RK_B[s] * self.dpi_o[c])
-                               for c in range(NCOMP)]
-            self.dn_o = dn
-            self.dmom_o = dmom
-            if self.stress.evolve:
-                self.dpi_o = dpi
-            dts = (RK_A[s] + RK_B[s]) * dt
-            ts = ts + dts
-            if self.stress.evolve:
-                self.velocity()
-                pi0 = self.stress.pi0(spc, self.u)
-                fac = math.exp(-dts / self.stress.tau)
-                phi = (self.stress.tau / dts) * (1.0 - fac)
-                self.pi = [pi0[c] - (pi0[c] - pi0_old[c]) * phi + (self.pi[c] - pi0_old[c]) * fac
-                           for c in range(NCOMP)]
-                self.pi = [spc.dealias(p) for p in self.pi]
-            self.n = spc.dealias(self.n)
-            for i in range(g.ndim):
-                self.mom[i] = spc.dealias(self.mom[i])
-        self.t = self.t + dt
+            self.advance_stage(s)
+        self.t = self.t + self.dt
         self.step += 1
+
+    def advance_stage(self, s):
+        """Stage s of model_advance (model.f90:192-223); the stage time is self._ts."""
+        spc, g = self.spc, self.spc.g
+        dt = self.dt
+        if s == 0:
+            self._ts = self.t
+        ts = self._ts
+        dn, dmom, dpi = self.rhs(ts)
+        if self.stress.evolve:
+            pi0_old = self.stress.pi0(spc, self.u)
+        if s == 0:
+            self.n = self.n + dt * RK_A[0] * dn
+            self.mom = [self.mom[i] + dt * RK_A[0] * dmom[i] for i in range(3)]
+            if self.stress.evolve:
+                self.pi = [self.pi[c] + dt * RK_A[0] * dpi[c] for c in range(NCOMP)]
+        else:
+            self.n = self.n + dt * (RK_A[s] * dn + RK_B[s] * self.dn_o)
+            self.mom = [self.mom[i] + dt * (RK_A[s] * dmom[i] + RK_B[s] * self.dmom_o[i]) for i in range(3)]
+            if self.stress.evolve:
+                self.pi = [self.pi[c] + dt * (RK_A[s] * dpi[c] + RK_B[s] * self.dpi_o[c])
+                           for c in range(NCOMP)]
+        self.dn_o = dn
+        self.dmom_o = dmom
+        if self.stress.evolve:
+            self.dpi_o = dpi
+        dts = (RK_A[s] + RK_B[s]) * dt
+        self._ts = ts + dts
+        if self.stress.evolve:
+            self.velocity()
+            pi0 = self.stress.pi0(spc, self.u)
+            fac = math.exp(-dts / self.stress.tau)
+            phi = (self.stress.tau / dts) * (1.0 - fac)
+            self.pi = [pi0[c] - (pi0[c] - pi0_old[c]) * phi + (self.pi[c] - pi0_old[c]) * fac
+                       for c in range(NCOMP)]
+            self.pi = [spc.dealias(p) for p in self.pi]
+        self.n = spc.dealias(self.n)
+        for i in range(g.ndim):
+            self.mom[i] = spc.dealias(self.mom[i])
 
     # diagnostics.f90:41-77
     def budgets(self):

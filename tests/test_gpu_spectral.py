@@ -74,18 +74,9 @@ SHAPES = [(32, 32, 32), (16, 12, 8), (48, 24, 4), (64, 1, 1), (32, 16, 1), (96, 
           (128, 64, 32)]
 
 
-# The first GPU run of the drop-in operators on the 32^3 grid with dealiasing
-# ended in a GPU memory fault.  The same sequence is clean under a CPU
-# emulation of the kernels with AddressSanitizer on every shape here, and the
-# cause has not been found; the case stays off the GPU until it is.
-FAULTED = {((32, 32, 32), True), ((32, 32, 32), False)}
-
-
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("dealias", [True, False])
 def test_ops_vs_oracle(Spec, shape, dealias):
-    if (tuple(shape), dealias) in FAULTED:
-        pytest.skip("faulted the GPU once; cause not found yet (CPU emulation clean)")
     nx, ny, nz = shape
     L = (20.0, 7.0, 3.0)
     s = Spec(nx, ny, nz, *L, dealias=dealias)
