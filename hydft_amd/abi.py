@@ -106,6 +106,10 @@ _SIGS = {
     "hy_model_probe": [_P, _I, _I, _I, _P],
     "hy_model_kernel": [_P, _D, _PD],
     "hy_fd_eval": [_I, _I, _P, _P],
+    "hy_device_alloc": [C.c_ulonglong, C.POINTER(_P)],
+    "hy_device_free": [_P],
+    "hy_copy_to_device": [_P, _P, C.c_ulonglong],
+    "hy_copy_to_host": [_P, _P, C.c_ulonglong],
     "hy_model_profile": [_P, _I],
     "hy_model_profile_read": [_P, _I, _P, _P, _P],
 }

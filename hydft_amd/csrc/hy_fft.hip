@@ -581,6 +581,28 @@ int hy_device_sync(void) {
   return HY_OK;
 }
 
+int hy_device_alloc(unsigned long long bytes, void** d_out) {
+  if (!d_out) return set_error(HY_ERR_INVALID, "null argument");
+  if (hipMalloc(d_out, bytes ? bytes : 1) != hipSuccess) {
+    (void)hipGetLastError();
+    *d_out = nullptr;
+    return set_error(HY_ERR_OOM, "hy_device_alloc: out of device memory");
+  }
+  return HY_OK;
+}
+int hy_device_free(void* d) {
+  if (d) HY_HIP(hipFree(d));
+  return HY_OK;
+}
+int hy_copy_to_device(void* d_dst, const void* h_src, unsigned long long bytes) {
+  HY_HIP(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+  return HY_OK;
+}
+int hy_copy_to_host(void* h_dst, const void* d_src, unsigned long long bytes) {
+  HY_HIP(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+  return HY_OK;
+}
+
 int hy_grid_create(int nx, int ny, int nz, double lx, double ly, double lz, int dealias,
                    int precision, hy_grid* out) {
   if (!out) return set_error(HY_ERR_INVALID, "hy_grid_create: null output");

@@ -205,7 +205,7 @@ struct PencilField {
   const void* in;
   void* out;
   int mul;          // MulKind, applied on the spectral side
-  int in_f64;       // input element type (1 = c64, 0 = c32); outputs are always the compute type
+  int in_f64;       // input element type: 1 = c64, 0 = c32 (outputs are the pass's compute type)
   double k0_offset; // added to the k = 0 entry on load (inverse first pass only)
 };
 constexpr int MAXPF = 24;

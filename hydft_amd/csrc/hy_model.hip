@@ -790,6 +790,9 @@ static void* zslot(Model& m, int i) { return (char*)m.zb + m.zsz * m.esz * i; }
 static void* fslot(Model& m, int i) { return (char*)m.fb + m.nt * m.esz * i; }
 static c64* sfield(Model& m, int f) { return m.S + m.nt * f; }
 
+// PencilField::in_f64 of a buffer holding the compute type
+static int ct64(const Model& m) { return m.esz == sizeof(c64) ? 1 : 0; }
+
 static SpecArgs spec_args(const Model& m) {
   SpecArgs a;
   const Grid& g = m.g;
@@ -839,8 +842,8 @@ static int inv_chain(Model& m, const std::vector<InvSpec>& fs, std::vector<XIn>&
     std::vector<PencilField> zf, yf;
     for (size_t i = 0; i < fs.size(); ++i) {
       const auto& f = fs[i];
-      zf.push_back({f.src, zslot(m, (int)i), f.mz, f.src64, f.k0off});
-      yf.push_back({zslot(m, (int)i), xslot(m, f.xslot), f.my, 0, 0.0});
+      zf.push_back({f.src, zslot(m, (int)i), f.mz, f.src64 ? 1 : ct64(m), f.k0off});
+      yf.push_back({zslot(m, (int)i), xslot(m, f.xslot), f.my, ct64(m), 0.0});
     }
     if ((int)fs.size() > m.nzslots) return set_error(HY_ERR_INVALID, "internal: z slots");
     prof_begin(m, PC_ZINV, pencil_bytes(m, zf, (size_t)g.Kx * g.Ky, 1, g.nz, g.Kz));
@@ -857,7 +860,7 @@ static int inv_chain(Model& m, const std::vector<InvSpec>& fs, std::vector<XIn>&
       for (size_t i = 0; i < fs.size(); ++i) xin[i].p = zslot(m, (int)i);
   } else if (g.ny > 1) {
     std::vector<PencilField> yf;
-    for (auto& f : fs) yf.push_back({f.src, xslot(m, f.xslot), f.my, f.src64, f.k0off});
+    for (auto& f : fs) yf.push_back({f.src, xslot(m, f.xslot), f.my, f.src64 ? 1 : ct64(m), f.k0off});
     prof_begin(m, PC_YINV, pencil_bytes(m, yf, g.Kx, 1, g.ny, g.Ky));
     HY_TRY(pencil_pass(g, 1, true, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st));
     prof_end(m);
@@ -880,9 +883,9 @@ static int fwd_chain(Model& m, const std::vector<std::pair<int, int>>& f /* (xsl
   if (g.nz > 1) {
     std::vector<PencilField> yf, zf;
     for (size_t i = 0; i < f.size(); ++i) {
-      if (g.ny > 1) yf.push_back({xslot(m, f[i].first), zslot(m, (int)i), MUL_NONE, 0, 0.0});
+      if (g.ny > 1) yf.push_back({xslot(m, f[i].first), zslot(m, (int)i), MUL_NONE, ct64(m), 0.0});
       const void* zin = g.ny > 1 ? zslot(m, (int)i) : xslot(m, f[i].first);
-      zf.push_back({zin, fslot(m, f[i].second), MUL_NONE, 0, 0.0});
+      zf.push_back({zin, fslot(m, f[i].second), MUL_NONE, ct64(m), 0.0});
     }
     if ((int)f.size() > m.nzslots) return set_error(HY_ERR_INVALID, "internal: z slots");
     if (g.ny > 1) {
@@ -895,7 +898,7 @@ static int fwd_chain(Model& m, const std::vector<std::pair<int, int>>& f /* (xsl
     prof_end(m);
   } else if (g.ny > 1) {
     std::vector<PencilField> yf;
-    for (auto& p : f) yf.push_back({xslot(m, p.first), fslot(m, p.second), MUL_NONE, 0, 0.0});
+    for (auto& p : f) yf.push_back({xslot(m, p.first), fslot(m, p.second), MUL_NONE, ct64(m), 0.0});
     prof_begin(m, PC_YFWD, pencil_bytes(m, yf, g.Kx, 1, g.ny, g.Ky));
     HY_TRY(pencil_pass(g, 1, false, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st));
     prof_end(m);

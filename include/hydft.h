@@ -47,6 +47,12 @@ const char* hy_last_error(void);
 int hy_version(void);
 int hy_device_sync(void);
 
+/* ---- device memory for callers without a device allocator (the Fortran host) */
+int hy_device_alloc(unsigned long long bytes, void** d_out);
+int hy_device_free(void* d);
+int hy_copy_to_device(void* d_dst, const void* h_src, unsigned long long bytes);
+int hy_copy_to_host(void* h_dst, const void* d_src, unsigned long long bytes);
+
 /* ---- grid_t + fft_t (src/core/grid.f90:34-113, src/core/fft.f90:36-49) -- */
 int hy_grid_create(int nx, int ny, int nz, double lx, double ly, double lz, int dealias,
                    int precision, hy_grid* out);
