@@ -1983,7 +1983,9 @@ int hy_model_budgets(hy_model h, double* out6) {
   double fe = 0.0;
   if (m.ideal) fe += (sum[1] / Nr) * V / m.beta;
   if (m.hartree) fe += 0.5 * (sum[2] / Nr) * V;
-  if (m.ry) fe += -0.5 * (sum[3] / Nr) * V / m.beta;
+  // w_ry holds B(K_ry N') with K_ry = -c_cor/beta, so -1/2 <(n-n0) c_cor*N'> V/beta
+  // (fe_correlation_ry.f90:96-97) is +1/2 <(n-n0) w_ry> V
+  if (m.ry) fe += 0.5 * (sum[3] / Nr) * V;
   if (m.tf) fe += (sum[4] / Nr) * V / m.beta;
   if (m.grad) fe += m.cg * (sum[5] / Nr) * V;
   if (m.vext_kind != HY_VEXT_NONE) fe += (sum[6] / Nr) * V;
