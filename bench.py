@@ -82,6 +82,19 @@ def cpu_baseline(p, st, n, mom):
             "sample": f"one RK3 stage (of 3) of the FP64 oracle at {p.nx}^3, {el:.1f} s, scaled to steps/s"}
 
 
+def copy_probe(mib):
+    """A plain copy kernel of known size: reads and writes mib MiB once per launch, 3 launches."""
+    import torch
+    x = torch.ones(mib * 262144, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": mib * 1048576, "copy_probe_bytes_written": mib * 1048576,
+                      "launches": 3}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +104,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; runs on more than one GPU are not part of it "
@@ -98,9 +113,10 @@ def main():
         return 1
 
     import torch
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     from hydft_amd import abi
     from hydft_amd.model import Model, build_species
-    from hydft_amd.host import hnc
 
     abi.call("hy_init", 0)
     p = config(args.n, args.precision)

@@ -121,5 +121,71 @@ __host__ __device__ inline double alpha_from_ip12(const double* tab, double y) {
   return alpha;
 }
 
+// I_{1/2}(a) and I_{-1/2}(a) from one node lookup (the pair every Newton
+// iteration needs; same interpolants as ip12 and im12).
+__host__ __device__ inline void ip12_im12(const double* tab, double a, double& ip, double& im) {
+  if (a > AMAX || a < AMIN) {
+    ip = ip12(tab, a);
+    im = im12(tab, a);
+    return;
+  }
+  const int i = node(a);
+  const double* r0 = tab + i * NC;
+  const double* r1 = r0 + NC;
+  const double t = (a - (AMIN + i * H)) / H;
+  const double t2 = t * t, t3 = t2 * t;
+  const double h00 = 2 * t3 - 3 * t2 + 1, h10 = (t3 - 2 * t2 + t) * H, h01 = -2 * t3 + 3 * t2,
+               h11 = (t3 - t2) * H;
+  const double m0 = r0[0], m1 = r1[0];
+  ip = h00 * r0[1] + h10 * (0.5 * m0) + h01 * r1[1] + h11 * (0.5 * m1);
+  im = h00 * m0 + h10 * r0[3] + h01 * m1 + h11 * r1[3];
+}
+
+// I_{-1/2}, I'_{-1/2}, I''_{-1/2} at a from one node lookup (the gradient term
+// needs all three: xi, d xi/d alpha and d n/d alpha).
+__host__ __device__ inline void im12_d1_d2(const double* tab, double a, double& im, double& d1, double& d2) {
+  if (a > AMAX || a < AMIN) {
+    im = im12(tab, a);
+    d1 = dim12(tab, a);
+    d2 = d2im12(tab, a);
+    return;
+  }
+  const int i = node(a);
+  const double* r0 = tab + i * NC;
+  const double* r1 = r0 + NC;
+  const double t = (a - (AMIN + i * H)) / H;
+  const double t2 = t * t, t3 = t2 * t;
+  const double h00 = 2 * t3 - 3 * t2 + 1, h10 = (t3 - 2 * t2 + t) * H, h01 = -2 * t3 + 3 * t2,
+               h11 = (t3 - t2) * H;
+  im = h00 * r0[0] + h10 * r0[3] + h01 * r1[0] + h11 * r1[3];
+  d1 = h00 * r0[3] + h10 * r0[4] + h01 * r1[3] + h11 * r1[4];
+  d2 = h00 * r0[4] + h10 * r0[5] + h01 * r1[4] + h11 * r1[5];
+}
+
+// alpha_from_ip12 started at `a_start` instead of the closed-form guess: the
+// same safeguarded Newton iteration and stopping rule, so it converges to the
+// same root to the iteration's 1e-14 relative step tolerance.  The field
+// kernels start it from the linearisation about the uniform state, which
+// needs two to three iterations for the small density excursions of a run.
+__host__ __device__ inline double alpha_from_ip12_from(const double* tab, double y, double a_start) {
+  const double huge = 1.7976931348623157e308;
+  double alpha = a_start;
+  double lo = -huge, hi = huge;
+  for (int it = 0; it < 100; ++it) {
+    double ip, im;
+    ip12_im12(tab, alpha, ip, im);
+    const double f = ip - y;
+    if (f > 0.0) hi = fmin(hi, alpha);
+    else lo = fmax(lo, alpha);
+    const double step = f / (0.5 * im);
+    if (fabs(step) < 1.0e-14 * fmax(1.0, fabs(alpha))) break;
+    alpha = alpha - step;
+    if (alpha <= lo || alpha >= hi) {
+      if (lo > -huge && hi < huge) alpha = 0.5 * (lo + hi);
+    }
+  }
+  return alpha;
+}
+
 }  // namespace fd
 }  // namespace hy

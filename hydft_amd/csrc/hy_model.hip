@@ -175,16 +175,17 @@ __device__ inline void x_inv_pair(cpx<T>* buf, const cpx<T>* tw, const XIn* A, c
   fft_lds<T, NX, 1, 64, true>(buf, tw, lane);
 }
 
-// Forward x transform of two real lines va + i vb; stores the first Kx
-// coefficients of each, scaled by 1/NX, to outA / outB (compute type).
-template <class T, int NX, int PTS>
-__device__ inline void x_fwd_pair(cpx<T>* buf, const cpx<T>* tw, const double* va, const double* vb,
-                                  void* outA, void* outB, size_t line, int Kx, int lane) {
+// Forward x transform of two real lines packed as va + i vb, where
+// val(p) = (va, vb) at the lane's p-th point x = lane + 64 p; stores the
+// first Kx coefficients of each, scaled by 1/NX, to outA / outB (compute type).
+template <class T, int NX, int PTS, class F>
+__device__ inline void x_fwd_pair(cpx<T>* buf, const cpx<T>* tw, F&& val, void* outA, void* outB,
+                                  size_t line, int Kx, int lane) {
   __syncthreads();
 #pragma unroll
   for (int p = 0; p < PTS; ++p) {
     int x = lane + 64 * p;
-    if (x < NX) buf[x] = mk<T>((T)va[p], vb ? (T)vb[p] : T(0));
+    if (x < NX) buf[x] = val(p);
   }
   __syncthreads();
   fft_lds<T, NX, 1, 64, false>(buf, tw, lane);
@@ -225,11 +226,17 @@ __device__ inline double wave_max(double v) {
 enum OutA { OA_MU = 0, OA_U1, OA_U2, OA_U3, OA_F11, OA_F12, OA_F13, OA_F22, OA_F23, OA_F33 };
 enum OutB { OB_H1 = 0, OB_H2, OB_H3, OB_ADV1 };  // ADV1..ADV6 = 3..8
 
+// Arguments of the x-line kernels.  They are written once per configuration
+// into device memory and the kernels read them through a pointer: a by-value
+// kernel argument whose arrays are indexed at run time, or whose members have
+// their address taken, is copied whole into per-lane scratch memory in every
+// workgroup.
 struct PhysArgs {
   int nd, Kx, ny, nz;
   double dx, dy, dz;
   const double* kx;
   double n0, mass, beta, cn, alpha0, cg;
+  double dadn;  // d alpha / d n at n0: start of the per-point Newton iteration
   int ideal, tf, grad;
   const double* fdtab;
   Ctl* ctl;
@@ -244,25 +251,27 @@ struct PhysArgs {
   double amp, ramp, omega, kvec[3];
   const double* fext;  // gaussian: 3 real fields
   double* red;         // per-line partial sums
-  // outputs
+  // outputs: phase A in the order of code[]; phase B: H_1..H_nd, then ADV_1..6
   int nout;
   int code[10];
   void* dst[10];
 };
 
 template <class T, int NX>
-__global__ __launch_bounds__(64) void k_phaseA(PhysArgs a, const cpx<T>* __restrict__ twg) {
+__global__ __launch_bounds__(64) void k_phaseA(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
   constexpr int PTS = NX >= 64 ? NX / 64 : 1;
   __shared__ cpx<T> buf[NX];
   __shared__ cpx<T> tw[NX];
+  const PhysArgs& a = *pa;
   const int lane = threadIdx.x;
   const size_t line = blockIdx.x;
+  const int nd = a.nd;
   for (int j = lane; j < NX; j += 64) tw[j] = twg[j];
   double dn[PTS], m0[PTS], m1[PTS], m2[PTS];
   x_inv_pair<T, NX>(buf, tw, &a.n, &a.m[0], line, a.Kx, a.kx, lane);
   x_get<T, NX, PTS>(buf, dn, m0, lane);
-  if (a.nd >= 2) {
-    x_inv_pair<T, NX>(buf, tw, &a.m[1], a.nd >= 3 ? &a.m[2] : nullptr, line, a.Kx, a.kx, lane);
+  if (nd >= 2) {
+    x_inv_pair<T, NX>(buf, tw, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, line, a.Kx, a.kx, lane);
     x_get<T, NX, PTS>(buf, m1, m2, lane);
   } else {
 #pragma unroll
@@ -270,191 +279,194 @@ __global__ __launch_bounds__(64) void k_phaseA(PhysArgs a, const cpx<T>* __restr
   }
   double mu[PTS], u0[PTS], u1[PTS], u2[PTS];
   double gx[PTS], gy[PTS], gz[PTS], lap[PTS];
+#pragma unroll
+  for (int p = 0; p < PTS; ++p) { gx[p] = 0.0; gy[p] = 0.0; gz[p] = 0.0; lap[p] = 0.0; }
   if (a.grad) {
-    x_inv_pair<T, NX>(buf, tw, &a.gx, a.nd >= 2 ? &a.gy : nullptr, line, a.Kx, a.kx, lane);
+    x_inv_pair<T, NX>(buf, tw, &a.gx, nd >= 2 ? &a.gy : nullptr, line, a.Kx, a.kx, lane);
     x_get<T, NX, PTS>(buf, gx, gy, lane);
-    x_inv_pair<T, NX>(buf, tw, a.nd >= 3 ? &a.gz : nullptr, &a.lap, line, a.Kx, a.kx, lane);
+    x_inv_pair<T, NX>(buf, tw, nd >= 3 ? &a.gz : nullptr, &a.lap, line, a.Kx, a.kx, lane);
     x_get<T, NX, PTS>(buf, gz, lap, lane);
   }
   bool bad = false;
 #pragma unroll
   for (int p = 0; p < PTS; ++p) {
     const bool live = (lane + 64 * p) < NX;
-    double n = a.n0 + dn[p];
+    const double n = a.n0 + dn[p];
     if (live && !(n > 0.0)) bad = true;
-    double inv = 1.0 / (a.mass * n);
+    const double inv = 1.0 / (a.mass * n);
     u0[p] = m0[p] * inv; u1[p] = m1[p] * inv; u2[p] = m2[p] * inv;
     // mu_loc with its k = 0 constants (ln n0 / beta, alpha0 / beta) dropped:
     // only grad(mu) enters the momentum equation (model.f90:99).
     double v = 0.0;
     if (a.ideal) v += log1p(dn[p] / a.n0) / a.beta;  // fe_ideal_classical.f90:40
     if (a.tf || a.grad) {
-      double al = live && n > 0.0 ? fd::alpha_from_ip12(a.fdtab, n / a.cn) : a.alpha0;
-      if (a.tf) v += (al - a.alpha0) / a.beta;       // fe_kinetic_tfk.f90:80-81
-      if (a.grad) {                                  // fe_kinetic_tfk.f90:130-137
-        double xi = fd::xi(a.fdtab, al);
-        double dxidn = fd::dxi(a.fdtab, al) / (0.5 * a.cn * fd::im12(a.fdtab, al));
-        double g2 = gx[p] * gx[p] + gy[p] * gy[p] + gz[p] * gz[p];
+      const double al = live && n > 0.0
+                            ? fd::alpha_from_ip12_from(a.fdtab, n / a.cn, a.alpha0 + a.dadn * dn[p])
+                            : a.alpha0;
+      if (a.tf) v += (al - a.alpha0) / a.beta;  // fe_kinetic_tfk.f90:80-81
+      if (a.grad) {                             // fe_kinetic_tfk.f90:130-137
+        double im, d1, d2;
+        fd::im12_d1_d2(a.fdtab, al, im, d1, d2);
+        const double xi = d1 / (im * im);
+        const double dxi = (d2 * im - 2.0 * d1 * d1) / (im * im * im);
+        const double dxidn = dxi / (0.5 * a.cn * im);
+        const double g2 = gx[p] * gx[p] + gy[p] * gy[p] + gz[p] * gz[p];
         v -= a.cg * (dxidn * g2 + 2.0 * xi * lap[p]);
       }
     }
     mu[p] = v;
   }
   if (bad) a.ctl->nonpos = 1;
-  const double* mm[3] = {m0, m1, m2};
-  const double* uu[3] = {u0, u1, u2};
-  for (int o = 0; o < a.nout; o += 2) {
-    double va[PTS], vb[PTS];
-    for (int q = 0; q < 2; ++q) {
-      if (o + q >= a.nout) break;
-      double* dst = q == 0 ? va : vb;
-      int c = a.code[o + q];
-#pragma unroll
-      for (int p = 0; p < PTS; ++p) {
-        double v;
-        switch (c) {
-          case OA_MU: v = mu[p]; break;
-          case OA_U1: v = u0[p]; break;
-          case OA_U2: v = u1[p]; break;
-          case OA_U3: v = u2[p]; break;
-          case OA_F11: v = mm[0][p] * uu[0][p]; break;
-          case OA_F12: v = mm[0][p] * uu[1][p]; break;
-          case OA_F13: v = mm[0][p] * uu[2][p]; break;
-          case OA_F22: v = mm[1][p] * uu[1][p]; break;
-          case OA_F23: v = mm[1][p] * uu[2][p]; break;
-          default: v = mm[2][p] * uu[2][p]; break;
-        }
-        dst[p] = v;
-      }
+  auto val = [&](int c, int p) -> double {
+    switch (c) {
+      case OA_MU: return mu[p];
+      case OA_U1: return u0[p];
+      case OA_U2: return u1[p];
+      case OA_U3: return u2[p];
+      case OA_F11: return m0[p] * u0[p];
+      case OA_F12: return m0[p] * u1[p];
+      case OA_F13: return m0[p] * u2[p];
+      case OA_F22: return m1[p] * u1[p];
+      case OA_F23: return m1[p] * u2[p];
+      case OA_F33: return m2[p] * u2[p];
+      default: return 0.0;
     }
+  };
+  for (int o = 0; o < a.nout; o += 2) {
     const bool two = o + 1 < a.nout;
-    x_fwd_pair<T, NX, PTS>(buf, tw, va, two ? vb : nullptr, a.dst[o], two ? a.dst[o + 1] : nullptr,
-                           line, a.Kx, lane);
+    const int ca = a.code[o], cb = two ? a.code[o + 1] : -1;
+    x_fwd_pair<T, NX, PTS>(buf, tw, [&](int p) { return mk<T>((T)val(ca, p), (T)val(cb, p)); }, a.dst[o],
+                           two ? a.dst[o + 1] : nullptr, line, a.Kx, lane);
   }
 }
 
 template <class T, int NX>
-__global__ __launch_bounds__(64) void k_phaseB(PhysArgs a, const cpx<T>* __restrict__ twg) {
+__global__ __launch_bounds__(64) void k_phaseB(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
   constexpr int PTS = NX >= 64 ? NX / 64 : 1;
   __shared__ cpx<T> buf[NX];
   __shared__ cpx<T> tw[NX];
+  const PhysArgs& a = *pa;
   const int lane = threadIdx.x;
   const size_t line = blockIdx.x;
+  const int nd = a.nd;
   const int iy = (int)(line % a.ny), iz = (int)(line / a.ny);
   for (int j = lane; j < NX; j += 64) tw[j] = twg[j];
   double dn[PTS], gmx[PTS], gmy[PTS], gmz[PTS];
   x_inv_pair<T, NX>(buf, tw, &a.n, &a.mux, line, a.Kx, a.kx, lane);
   x_get<T, NX, PTS>(buf, dn, gmx, lane);
-  if (a.nd >= 2) {
-    x_inv_pair<T, NX>(buf, tw, &a.muy, a.nd >= 3 ? &a.muz : nullptr, line, a.Kx, a.kx, lane);
+  if (nd >= 2) {
+    x_inv_pair<T, NX>(buf, tw, &a.muy, nd >= 3 ? &a.muz : nullptr, line, a.Kx, a.kx, lane);
     x_get<T, NX, PTS>(buf, gmy, gmz, lane);
   } else {
 #pragma unroll
     for (int p = 0; p < PTS; ++p) { gmy[p] = 0.0; gmz[p] = 0.0; }
   }
   // external force (external_potential.f90:65-93), at the stage-start time
-  double f[3][PTS];
-  double sf[3] = {0.0, 0.0, 0.0};
+  double f0[PTS], f1[PTS], f2[PTS];
 #pragma unroll
-  for (int p = 0; p < PTS; ++p) { f[0][p] = 0.0; f[1][p] = 0.0; f[2][p] = 0.0; }
+  for (int p = 0; p < PTS; ++p) { f0[p] = 0.0; f1[p] = 0.0; f2[p] = 0.0; }
   if (a.vext_kind == HY_VEXT_MODE || a.vext_kind == HY_VEXT_DRIVEN) {
-    double t = a.vext_td ? a.ctl->ts : 0.0;
+    const double t = a.vext_td ? a.ctl->ts : 0.0;
     double amp = a.amp;
     if (a.vext_td && a.ramp > 0.0) amp = amp * fmin(1.0, t / a.ramp);
-    double yz = a.kvec[1] * (iy * a.dy) + a.kvec[2] * (iz * a.dz);
+    const double yz = a.kvec[1] * (iy * a.dy) + a.kvec[2] * (iz * a.dz);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) {
-      int x = lane + 64 * p;
-      double ph = a.kvec[0] * (x * a.dx) + yz - a.omega * t;
-      double s = amp * sin(ph);
-      f[0][p] = s * a.kvec[0]; f[1][p] = s * a.kvec[1]; f[2][p] = s * a.kvec[2];
+      const int x = lane + 64 * p;
+      const double s = amp * sin(a.kvec[0] * (x * a.dx) + yz - a.omega * t);
+      f0[p] = s * a.kvec[0]; f1[p] = s * a.kvec[1]; f2[p] = s * a.kvec[2];
     }
   } else if (a.vext_kind == HY_VEXT_GAUSSIAN) {
     const size_t nr = (size_t)NX * a.ny * a.nz;
+    const double* fe = a.fext + line * NX;
 #pragma unroll
     for (int p = 0; p < PTS; ++p) {
-      int x = lane + 64 * p;
-      if (x < NX)
-        for (int d = 0; d < 3; ++d) f[d][p] = a.fext[d * nr + line * NX + x];
+      const int x = lane + 64 * p;
+      if (x < NX) { f0[p] = fe[x]; f1[p] = fe[nr + x]; f2[p] = fe[2 * nr + x]; }
     }
   }
-  double h[3][PTS];
+  double h0[PTS], h1[PTS], h2[PTS];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
   for (int p = 0; p < PTS; ++p) {
     const bool live = (lane + 64 * p) < NX;
-    double n = a.n0 + dn[p];
-    h[0][p] = n * (f[0][p] - gmx[p]);
-    h[1][p] = n * (f[1][p] - gmy[p]);
-    h[2][p] = n * (f[2][p] - gmz[p]);
-    if (live) { sf[0] += n * f[0][p]; sf[1] += n * f[1][p]; sf[2] += n * f[2][p]; }
+    const double n = a.n0 + dn[p];
+    h0[p] = n * (f0[p] - gmx[p]);
+    h1[p] = n * (f1[p] - gmy[p]);
+    h2[p] = n * (f2[p] - gmz[p]);
+    if (live) { s0 += n * f0[p]; s1 += n * f1[p]; s2 += n * f2[p]; }
   }
   if (a.vext_kind != HY_VEXT_NONE) {
-    for (int d = 0; d < 3; ++d) {
-      double s = wave_sum(sf[d]);
-      if (lane == 0) a.red[line * 3 + d] = s;
-    }
+    s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2);
+    if (lane == 0) { a.red[line * 3] = s0; a.red[line * 3 + 1] = s1; a.red[line * 3 + 2] = s2; }
   }
-  // Maxwell stress advection dPi_c = -sum_d u_d d_d Pi_c (model.f90:126-134)
-  double adv[6][PTS];
-  if (a.evolve) {
-    double m0[PTS], m1[PTS], m2[PTS], u[3][PTS];
-    x_inv_pair<T, NX>(buf, tw, &a.m[0], a.nd >= 2 ? &a.m[1] : nullptr, line, a.Kx, a.kx, lane);
+  auto hval = [&](int c, int p) -> double { return c == 0 ? h0[p] : (c == 1 ? h1[p] : (c == 2 ? h2[p] : 0.0)); };
+  for (int o = 0; o < nd; o += 2) {
+    const bool two = o + 1 < nd;
+    x_fwd_pair<T, NX, PTS>(buf, tw, [&](int p) { return mk<T>((T)hval(o, p), (T)hval(two ? o + 1 : -1, p)); },
+                           a.dst[o], two ? a.dst[o + 1] : nullptr, line, a.Kx, lane);
+  }
+  if (!a.evolve) return;
+  // Maxwell stress advection dPi_c = -sum_d u_d d_d Pi_c (model.f90:126-134),
+  // two components per pass
+  double u0[PTS], u1[PTS], u2[PTS];
+  {
+    double m0[PTS], m1[PTS], m2[PTS];
+    x_inv_pair<T, NX>(buf, tw, &a.m[0], nd >= 2 ? &a.m[1] : nullptr, line, a.Kx, a.kx, lane);
     x_get<T, NX, PTS>(buf, m0, m1, lane);
-    if (a.nd >= 3) {
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) m2[p] = 0.0;
+    if (nd >= 3) {
       x_inv_pair<T, NX>(buf, tw, &a.m[2], nullptr, line, a.Kx, a.kx, lane);
       x_get<T, NX, PTS>(buf, m2, nullptr, lane);
-    } else {
-#pragma unroll
-      for (int p = 0; p < PTS; ++p) m2[p] = 0.0;
     }
 #pragma unroll
     for (int p = 0; p < PTS; ++p) {
-      double inv = 1.0 / (a.mass * (a.n0 + dn[p]));
-      u[0][p] = m0[p] * inv; u[1][p] = m1[p] * inv; u[2][p] = m2[p] * inv;
-    }
-    for (int c = 0; c < 6; ++c) {
-      double dx_[PTS], dy_[PTS], dz_[PTS];
-      x_inv_pair<T, NX>(buf, tw, &a.px[c], a.nd >= 2 ? &a.py[c] : nullptr, line, a.Kx, a.kx, lane);
-      x_get<T, NX, PTS>(buf, dx_, dy_, lane);
-      if (a.nd >= 3) {
-        x_inv_pair<T, NX>(buf, tw, &a.pz[c], nullptr, line, a.Kx, a.kx, lane);
-        x_get<T, NX, PTS>(buf, dz_, nullptr, lane);
-      }
-#pragma unroll
-      for (int p = 0; p < PTS; ++p) {
-        double v = 0.0;
-        v = v - u[0][p] * dx_[p];
-        if (a.nd >= 2) v = v - u[1][p] * dy_[p];
-        if (a.nd >= 3) v = v - u[2][p] * dz_[p];
-        adv[c][p] = v;
-      }
+      const double inv = 1.0 / (a.mass * (a.n0 + dn[p]));
+      u0[p] = m0[p] * inv; u1[p] = nd >= 2 ? m1[p] * inv : 0.0; u2[p] = m2[p] * inv;
     }
   }
-  for (int o = 0; o < a.nout; o += 2) {
-    double va[PTS], vb[PTS];
-    for (int q = 0; q < 2; ++q) {
-      if (o + q >= a.nout) break;
-      double* dst = q == 0 ? va : vb;
-      int c = a.code[o + q];
+  for (int c = 0; c < 6; c += 2) {
+    double xa[PTS], ya[PTS], za[PTS], xb[PTS], yb[PTS], zb[PTS];
 #pragma unroll
-      for (int p = 0; p < PTS; ++p) dst[p] = c < 3 ? h[c][p] : adv[c - 3][p];
+    for (int p = 0; p < PTS; ++p) { ya[p] = 0.0; za[p] = 0.0; yb[p] = 0.0; zb[p] = 0.0; }
+    if (nd == 1) {
+      x_inv_pair<T, NX>(buf, tw, &a.px[c], &a.px[c + 1], line, a.Kx, a.kx, lane);
+      x_get<T, NX, PTS>(buf, xa, xb, lane);
+    } else {
+      x_inv_pair<T, NX>(buf, tw, &a.px[c], &a.py[c], line, a.Kx, a.kx, lane);
+      x_get<T, NX, PTS>(buf, xa, ya, lane);
+      x_inv_pair<T, NX>(buf, tw, &a.px[c + 1], &a.py[c + 1], line, a.Kx, a.kx, lane);
+      x_get<T, NX, PTS>(buf, xb, yb, lane);
+      if (nd >= 3) {
+        x_inv_pair<T, NX>(buf, tw, &a.pz[c], &a.pz[c + 1], line, a.Kx, a.kx, lane);
+        x_get<T, NX, PTS>(buf, za, zb, lane);
+      }
     }
-    const bool two = o + 1 < a.nout;
-    x_fwd_pair<T, NX, PTS>(buf, tw, va, two ? vb : nullptr, a.dst[o], two ? a.dst[o + 1] : nullptr,
-                           line, a.Kx, lane);
+    x_fwd_pair<T, NX, PTS>(
+        buf, tw,
+        [&](int p) {
+          const double va = -(u0[p] * xa[p]) - u1[p] * ya[p] - u2[p] * za[p];
+          const double vb = -(u0[p] * xb[p]) - u1[p] * yb[p] - u2[p] * zb[p];
+          return mk<T>((T)va, (T)vb);
+        },
+        a.dst[nd + c], a.dst[nd + c + 1], line, a.Kx, lane);
   }
 }
 
 // max |u_d| over a line (model.f90:171-172)
 template <class T, int NX>
-__global__ __launch_bounds__(64) void k_umax(PhysArgs a, const cpx<T>* __restrict__ twg) {
+__global__ __launch_bounds__(64) void k_umax(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
   constexpr int PTS = NX >= 64 ? NX / 64 : 1;
   __shared__ cpx<T> buf[NX];
   __shared__ cpx<T> tw[NX];
+  const PhysArgs& a = *pa;
   const int lane = threadIdx.x;
   const size_t line = blockIdx.x;
   for (int j = lane; j < NX; j += 64) tw[j] = twg[j];
   double dn[PTS], m0[PTS], m1[PTS], m2[PTS];
+#pragma unroll
+  for (int p = 0; p < PTS; ++p) { m1[p] = 0.0; m2[p] = 0.0; }
   x_inv_pair<T, NX>(buf, tw, &a.n, &a.m[0], line, a.Kx, a.kx, lane);
   x_get<T, NX, PTS>(buf, dn, m0, lane);
   if (a.nd >= 2) {
@@ -465,7 +477,7 @@ __global__ __launch_bounds__(64) void k_umax(PhysArgs a, const cpx<T>* __restric
 #pragma unroll
   for (int p = 0; p < PTS; ++p) {
     if (lane + 64 * p >= NX) continue;
-    double inv = 1.0 / (a.mass * (a.n0 + dn[p]));
+    const double inv = 1.0 / (a.mass * (a.n0 + dn[p]));
     um = fmax(um, fabs(m0[p] * inv));
     if (a.nd >= 2) um = fmax(um, fabs(m1[p] * inv));
     if (a.nd >= 3) um = fmax(um, fabs(m2[p] * inv));
@@ -473,7 +485,6 @@ __global__ __launch_bounds__(64) void k_umax(PhysArgs a, const cpx<T>* __restric
   um = wave_max(um);
   if (lane == 0) a.red[line] = um;
 }
-
 // compute_dt from the per-line maxima (model.f90:169-181)
 __global__ void k_dt(const double* __restrict__ red, int nlines, Ctl* c, double wmax, double numax,
                      double dxmin, double cfl) {
@@ -610,46 +621,54 @@ struct K2Args {
   const Ctl* ctl;
 };
 
-template <class T>
+// ND is a template parameter so that every loop below unrolls and every
+// array index is a constant: a run-time index into the by-value K2Args (or
+// into a local array) would put them in per-lane scratch memory.
+template <class T, int ND>
 __global__ void k_spec2(K2Args a) {
   const size_t nt = a.s.nt;
-  const int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-  const int idx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};  // stress.f90:92
-  const int nd = a.s.nd;
+  constexpr int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+  constexpr int idx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};  // stress.f90:92
   const double dt = a.ctl->dt;
   const double ra = c_rk_a[a.stage], rb = c_rk_b[a.stage];
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nt; e += (size_t)gridDim.x * blockDim.x) {
     double k[3];
     spec_k(a.s, e, k);
-    c64 M[3] = {a.S[nt + e], a.S[2 * nt + e], a.S[3 * nt + e]};
-    c64 r[10];
     // continuity: dn = -div(mom)/m (model.f90:95-96)
-    c64 km = scal(M[0], k[0]) + scal(M[1], k[1]) + scal(M[2], k[2]);
+    c64 km = mk<double>(0, 0);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) km = km + scal(a.S[(1 + d) * nt + e], k[d]);
+    c64 r[10];
     r[0] = mk<double>(km.y / a.mass, -km.x / a.mass);
     // stress divergence f_i = d_j Pi_ij
     c64 pih[6];
-    bool have_pi = false;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) pih[c] = mk<double>(0, 0);
+    const bool have_pi = a.evolve || a.stress_newt;
     if (a.evolve) {
+#pragma unroll
       for (int c = 0; c < 6; ++c) pih[c] = a.S[(4 + c) * nt + e];
-      have_pi = true;
     } else if (a.stress_newt) {
       c64 u[3];
-      for (int d = 0; d < 3; ++d) u[d] = d < nd ? ld<T>(a.fu[d], e) : mk<double>(0, 0);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) u[d] = d < ND ? ld<T>(a.fu[d], e) : mk<double>(0, 0);
       pi0_hat(u, k, a.eta, a.xi, pih);
-      have_pi = true;
     }
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
       c64 acc = mk<double>(0, 0);
-      if (i < nd) {
+      if (i < ND) {
         acc = ld<T>(a.fh[i], e);  // F(n (f_ext - d_i mu))
-        for (int j = 0; j < nd; ++j) {  // - i k_j F(m_i u_j)
-          c64 fv = ld<T>(a.fl[sym[i][j]], e);
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {  // - i k_j F(m_i u_j)
+          const c64 fv = ld<T>(a.fl[sym[i][j]], e);
           acc.x += k[j] * fv.y;
           acc.y -= k[j] * fv.x;
         }
         if (have_pi) {
-          for (int j = 0; j < nd; ++j) {
-            c64 pv = pih[idx[i][j]];
+#pragma unroll
+          for (int j = 0; j < ND; ++j) {
+            const c64 pv = pih[idx[i][j]];
             acc.x -= k[j] * pv.y;
             acc.y += k[j] * pv.x;
           }
@@ -660,25 +679,30 @@ __global__ void k_spec2(K2Args a) {
       }
       r[1 + i] = acc;
     }
-    int nf = 4;
     if (a.evolve) {
+#pragma unroll
       for (int c = 0; c < 6; ++c) r[4 + c] = ld<T>(a.fadv[c], e);
-      nf = 10;
     }
-    if (a.rhs_only) {
-      for (int f = 0; f < nf; ++f) a.R[f * nt + e] = r[f];
-      continue;
-    }
-    for (int f = 0; f < nf; ++f) {
-      c64 s = a.S[f * nt + e];
-      if (a.stage == 0) {
-        s = s + scal(r[f], dt * ra);
-      } else {
-        c64 ro = a.R[f * nt + e];
-        s = s + scal(scal(r[f], ra) + scal(ro, rb), dt);
+    auto put = [&](int f, c64 rf) {
+      if (a.rhs_only) {
+        a.R[f * nt + e] = rf;
+        return;
       }
-      a.S[f * nt + e] = s;
-      a.R[f * nt + e] = r[f];
+      c64 sv = a.S[f * nt + e];
+      if (a.stage == 0) {
+        sv = sv + scal(rf, dt * ra);
+      } else {
+        const c64 ro = a.R[f * nt + e];
+        sv = sv + scal(scal(rf, ra) + scal(ro, rb), dt);
+      }
+      a.S[f * nt + e] = sv;
+      a.R[f * nt + e] = rf;
+    };
+#pragma unroll
+    for (int f = 0; f < 4; ++f) put(f, r[f]);
+    if (a.evolve) {
+#pragma unroll
+      for (int f = 4; f < 10; ++f) put(f, r[f]);
     }
   }
 }
@@ -724,6 +748,7 @@ struct Model {
   double mass = 1, n0 = 0;
   int ideal = 0, tf = 0, grad = 0, hartree = 0, ry = 0;
   double beta = 3, cn = 0, alpha0 = 0, cg = 0, e2 = 1.0 / 3.0, kappa = 0;
+  double dadn = 0;  // 1 / (dn/dalpha) at alpha0: 2 / (cn I_{-1/2}(alpha0))
   std::vector<hy_term_desc> terms;
   std::vector<std::vector<double>> ktabs, ctabs;
   int closure = 0, evolve = 0, stress_newt = 0;
@@ -745,6 +770,9 @@ struct Model {
   void* fb = nullptr;   // truncated spectra, each nt
   double* red = nullptr;
   Ctl* ctl = nullptr;
+  PhysArgs* d_xargs = nullptr;  // XA_N device copies of the x-kernel arguments
+  PhysArgs h_xargs[4];          // what each device copy holds
+  bool xargs_ok[4] = {false, false, false, false};
   double* probe = nullptr;
   int probe_cap = 0;
   int nxslots = 0, nzslots = 0, nfslots = 0;
@@ -908,14 +936,16 @@ static int fwd_chain(Model& m, const std::vector<std::pair<int, int>>& f /* (xsl
 
 // ---- x kernel dispatch
 enum XK { XK_A = 0, XK_B = 1, XK_UMAX = 2 };
+// device argument slots: one per distinct argument set
+enum XSlot { XA_STAGE = 0, XA_RELAX = 1, XB_STAGE = 2, XU_DT = 3, XA_N = 4 };
 template <class T, int NX>
-static void launch_xk(int which, const PhysArgs& a, int nlines, const cpx<T>* tw, hipStream_t s) {
+static void launch_xk(int which, const PhysArgs* a, int nlines, const cpx<T>* tw, hipStream_t s) {
   if (which == XK_A) hipLaunchKernelGGL((k_phaseA<T, NX>), dim3(nlines), dim3(64), 0, s, a, tw);
   else if (which == XK_B) hipLaunchKernelGGL((k_phaseB<T, NX>), dim3(nlines), dim3(64), 0, s, a, tw);
   else hipLaunchKernelGGL((k_umax<T, NX>), dim3(nlines), dim3(64), 0, s, a, tw);
 }
 template <class T>
-static int dispatch_xk(int which, const Model& m, const PhysArgs& a, const cpx<T>* tw) {
+static int dispatch_xk(int which, const Model& m, const PhysArgs* a, const cpx<T>* tw) {
   const int nlines = m.g.ny * m.g.nz;
   switch (m.g.nx) {
 #define HY_XK(LL) case LL: launch_xk<T, LL>(which, a, nlines, tw, m.st); break;
@@ -944,10 +974,25 @@ static double xk_bytes(const Model& m, const PhysArgs& a) {
   b += line * a.nout * (double)m.esz;
   return b;
 }
-static int run_xk(Model& m, int which, const PhysArgs& a) {
+// Device copy of an argument set.  The sets depend only on the model's
+// configuration, so each slot is uploaded on its first use; a changed set is
+// uploaded after the stream has drained (no launch still reads the old one).
+static int xargs_dev(Model& m, int slot, const PhysArgs& a, const PhysArgs** out) {
+  if (!m.xargs_ok[slot] || std::memcmp(&m.h_xargs[slot], &a, sizeof(PhysArgs)) != 0) {
+    HY_HIP(hipStreamSynchronize(m.st));
+    HY_HIP(hipMemcpy(m.d_xargs + slot, &a, sizeof(PhysArgs), hipMemcpyHostToDevice));
+    std::memcpy(&m.h_xargs[slot], &a, sizeof(PhysArgs));
+    m.xargs_ok[slot] = true;
+  }
+  *out = m.d_xargs + slot;
+  return HY_OK;
+}
+static int run_xk(Model& m, int which, int slot, const PhysArgs& a) {
   const TwiddleSet& t = m.g.tw(m.g.nx);
+  const PhysArgs* da = nullptr;
+  HY_TRY(xargs_dev(m, slot, a, &da));
   prof_begin(m, which == XK_A ? PC_XA : (which == XK_B ? PC_XB : PC_UMAX), xk_bytes(m, a));
-  int rc = mixed(m) ? dispatch_xk<float>(which, m, a, t.d32) : dispatch_xk<double>(which, m, a, t.d64);
+  int rc = mixed(m) ? dispatch_xk<float>(which, m, da, t.d32) : dispatch_xk<double>(which, m, da, t.d64);
   prof_end(m);
   return rc;
 }
@@ -959,6 +1004,7 @@ static PhysArgs phys_base(Model& m) {
   a.nd = g.ndim; a.Kx = g.Kx; a.ny = g.ny; a.nz = g.nz;
   a.dx = g.dx; a.dy = g.dy; a.dz = g.dz; a.kx = g.d_kx;
   a.n0 = m.n0; a.mass = m.mass; a.beta = m.beta; a.cn = m.cn; a.alpha0 = m.alpha0; a.cg = m.cg;
+  a.dadn = m.dadn;
   a.ideal = m.ideal; a.tf = m.tf; a.grad = m.grad;
   a.fdtab = m.fdtab; a.ctl = m.ctl; a.evolve = m.evolve;
   a.vext_kind = m.vext_kind; a.vext_td = m.vext_td; a.amp = m.vamp; a.ramp = m.vramp;
@@ -1021,7 +1067,7 @@ static int stage(Model& m, int s, int rhs_only) {
   const int fcode[3][3] = {{OA_F11, OA_F12, OA_F13}, {OA_F12, OA_F22, OA_F23}, {OA_F13, OA_F23, OA_F33}};
   for (int i = 0; i < nd; ++i)
     for (int j = i; j < nd; ++j) addA(fcode[i][j]);
-  HY_TRY(run_xk(m, XK_A, a));
+  HY_TRY(run_xk(m, XK_A, XA_STAGE, a));
   HY_TRY(fwd_chain(m, fwd));
   // ---- K1: mu_hat (+ pending Maxwell relaxation)
   HY_TRY(launch_k1(m, m.relax_pending && !rhs_only));
@@ -1063,7 +1109,7 @@ static int stage(Model& m, int s, int rhs_only) {
   };
   for (int i = 0; i < nd; ++i) addB(OB_H1 + i);
   if (m.evolve) for (int c = 0; c < 6; ++c) addB(OB_ADV1 + c);
-  HY_TRY(run_xk(m, XK_B, b));
+  HY_TRY(run_xk(m, XK_B, XB_STAGE, b));
   HY_TRY(fwd_chain(m, fwdb));
   if (m.vext_kind != HY_VEXT_NONE) {
     hipLaunchKernelGGL(k_reduce_lines, dim3(1), dim3(256), 0, m.st, m.red, g.ny * g.nz, 3,
@@ -1131,8 +1177,15 @@ static int launch_k2(Model& m, int stage_idx, int rhs_only) {
     b += nt * 16.0 * (ns * 2 + (rhs_only ? ns : ns * 2));
     prof_begin(m, PC_K2, b);
   }
-  if (mixed(m)) hipLaunchKernelGGL(k_spec2<float>, dim3(nblocks(m.nt)), dim3(256), 0, m.st, a);
-  else hipLaunchKernelGGL(k_spec2<double>, dim3(nblocks(m.nt)), dim3(256), 0, m.st, a);
+  const dim3 gr(nblocks(m.nt)), bl(256);
+  switch (m.g.ndim * 2 + (mixed(m) ? 1 : 0)) {
+    case 2: hipLaunchKernelGGL((k_spec2<double, 1>), gr, bl, 0, m.st, a); break;
+    case 3: hipLaunchKernelGGL((k_spec2<float, 1>), gr, bl, 0, m.st, a); break;
+    case 4: hipLaunchKernelGGL((k_spec2<double, 2>), gr, bl, 0, m.st, a); break;
+    case 5: hipLaunchKernelGGL((k_spec2<float, 2>), gr, bl, 0, m.st, a); break;
+    case 6: hipLaunchKernelGGL((k_spec2<double, 3>), gr, bl, 0, m.st, a); break;
+    default: hipLaunchKernelGGL((k_spec2<float, 3>), gr, bl, 0, m.st, a); break;
+  }
   prof_end(m);
   HY_HIP(hipGetLastError());
   HY_TRACE("spec_update", m.st);
@@ -1153,7 +1206,7 @@ static int finish_relax(Model& m) {
     a.dst[o] = fwd_dst(m, XS_A0 + o, FS_A(OA_U1 + d));
     fwd.push_back({XS_A0 + o, FS_A(OA_U1 + d)});
   }
-  HY_TRY(run_xk(m, XK_A, a));
+  HY_TRY(run_xk(m, XK_A, XA_RELAX, a));
   HY_TRY(fwd_chain(m, fwd));
   // mu is not recomputed here: K1 rewrites mu_hat from the stale F(mu_loc) but
   // it is never read before the next stage recomputes it.
@@ -1174,7 +1227,7 @@ static int compute_dt_dev(Model& m) {
   }
   PhysArgs a = phys_base(m);
   HY_TRY(stage_inputs(m, a, false));
-  HY_TRY(run_xk(m, XK_UMAX, a));
+  HY_TRY(run_xk(m, XK_UMAX, XU_DT, a));
   hipLaunchKernelGGL(k_dt, dim3(1), dim3(256), 0, m.st, m.red, m.g.ny * m.g.nz, m.ctl, m.wmax,
                      m.numax, m.dxmin, m.cfl);
   HY_HIP(hipGetLastError());
@@ -1248,7 +1301,7 @@ static double fun_kernel(const Model& m, double k) {
 
 static void model_free(Model& m) {
   for (void* p : {(void*)m.S, (void*)m.R, (void*)m.uold, (void*)m.kcomb, (void*)m.fext, (void*)m.fdtab,
-                  m.xb, m.zb, m.fb, (void*)m.red, (void*)m.ctl, (void*)m.probe})
+                  m.xb, m.zb, m.fb, (void*)m.red, (void*)m.ctl, (void*)m.probe, (void*)m.d_xargs})
     if (p) (void)hipFree(p);
   for (auto& e : m.evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
   for (auto e : m.pool) (void)hipEventDestroy(e);
@@ -1603,7 +1656,9 @@ int model_build(Model& m, const hy_model_desc* d) {
     const double* tab = fd::host_table();
     HY_TRY(dmalloc((void**)&m.fdtab, sizeof(double) * fd::NTAB * fd::NC));
     HY_HIP(hipMemcpy(m.fdtab, tab, sizeof(double) * fd::NTAB * fd::NC, hipMemcpyHostToDevice));
+    m.dadn = 2.0 / (m.cn * fd::im12(tab, m.alpha0));
   }
+  HY_TRY(dmalloc((void**)&m.d_xargs, sizeof(PhysArgs) * XA_N));
   // combined convolution kernel on the kept box, and compute_dt constants
   std::vector<double> kc(m.nt, 0.0);
   double wmax = 0.0;
