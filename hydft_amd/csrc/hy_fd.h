@@ -9,11 +9,13 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 
 namespace hy {
 namespace fd {
 
 constexpr double AMIN = -30.0, AMAX = 300.0, H = 0.05;
+constexpr double INVH = 20.0;  // 1/H, exact: the node search multiplies instead of dividing
 constexpr int NTAB = 6601;  // nint((AMAX - AMIN)/H) + 1
 constexpr int NC = 6;       // columns: I-1/2, I1/2, I3/2, I'-1/2, I''-1/2, I'''-1/2
 constexpr double PI = 3.14159265358979323846264338327950288;
@@ -23,21 +25,23 @@ constexpr double SQRTPI = 1.77245385090551602729816748334114518;
 const double* host_table();
 
 __host__ __device__ inline int node(double a) {
-  int i = (int)((a - AMIN) / H);
+  int i = (int)((a - AMIN) * INVH);
   return i < 0 ? 0 : (i > NTAB - 2 ? NTAB - 2 : i);
 }
 
 // cubic Hermite of column `col` with derivative values d0, d1 at the nodes i, i+1
 __host__ __device__ inline double hermite(const double* tab, int col, double a, int i, double d0, double d1) {
-  double t = (a - (AMIN + i * H)) / H;
+  double t = (a - (AMIN + i * H)) * INVH;
   double t2 = t * t, t3 = t2 * t;
   double h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + t, h01 = -2 * t3 + 3 * t2, h11 = t3 - t2;
   double y0 = tab[i * NC + col], y1 = tab[(i + 1) * NC + col];
   return h00 * y0 + h10 * H * d0 + h01 * y1 + h11 * H * d1;
 }
 
-// m-th alpha-derivative of I_p from the Sommerfeld expansion (alpha >> 1)
-__host__ __device__ inline double sommerfeld(double p, double a, int m) {
+// m-th alpha-derivative of I_p from the Sommerfeld expansion (alpha >> 1).
+// The tails are out of line: they are rarely taken, and inlined they would
+// dominate the register allocation of the field kernels.
+__host__ __device__ __attribute__((noinline)) double sommerfeld(double p, double a, int m) {
   double c[3] = {1.0, (PI * PI / 6.0) * p * (p + 1.0),
                  (7.0 * PI * PI * PI * PI / 360.0) * (p + 1.0) * p * (p - 1.0) * (p - 2.0)};
   double v = 0.0;
@@ -51,7 +55,7 @@ __host__ __device__ inline double sommerfeld(double p, double a, int m) {
 }
 
 // m-th derivative of I_p from the classical expansion (alpha << -1); gp1 = Gamma(p+1)
-__host__ __device__ inline double classical(double p, double a, int m, double gp1) {
+__host__ __device__ __attribute__((noinline)) double classical(double p, double a, int m, double gp1) {
   double v = 0.0, sgn = 1.0;
   for (int j = 1; j <= 12; ++j) {
     v += sgn * exp(j * a) * pow((double)j, m - p - 1.0);
@@ -132,7 +136,7 @@ __host__ __device__ inline void ip12_im12(const double* tab, double a, double& i
   const int i = node(a);
   const double* r0 = tab + i * NC;
   const double* r1 = r0 + NC;
-  const double t = (a - (AMIN + i * H)) / H;
+  const double t = (a - (AMIN + i * H)) * INVH;
   const double t2 = t * t, t3 = t2 * t;
   const double h00 = 2 * t3 - 3 * t2 + 1, h10 = (t3 - 2 * t2 + t) * H, h01 = -2 * t3 + 3 * t2,
                h11 = (t3 - t2) * H;
@@ -153,7 +157,7 @@ __host__ __device__ inline void im12_d1_d2(const double* tab, double a, double& 
   const int i = node(a);
   const double* r0 = tab + i * NC;
   const double* r1 = r0 + NC;
-  const double t = (a - (AMIN + i * H)) / H;
+  const double t = (a - (AMIN + i * H)) * INVH;
   const double t2 = t * t, t3 = t2 * t;
   const double h00 = 2 * t3 - 3 * t2 + 1, h10 = (t3 - 2 * t2 + t) * H, h01 = -2 * t3 + 3 * t2,
                h11 = (t3 - t2) * H;
@@ -178,6 +182,92 @@ __host__ __device__ inline double alpha_from_ip12_from(const double* tab, double
     if (f > 0.0) hi = fmin(hi, alpha);
     else lo = fmax(lo, alpha);
     const double step = f / (0.5 * im);
+    if (fabs(step) < 1.0e-14 * fmax(1.0, fabs(alpha))) break;
+    alpha = alpha - step;
+    if (alpha <= lo || alpha >= hi) {
+      if (lo > -huge && hi < huge) alpha = 0.5 * (lo + hi);
+    }
+  }
+  return alpha;
+}
+
+// ---------------------------------------------------------------- device batch
+// Device-side table access through the global address space with 32-bit
+// offsets (scalar base + one vector offset per gather).
+__device__ inline double tabd(const double* tab, int idx) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(1))) const char gchar;
+  typedef __attribute__((address_space(1))) const double gdbl;
+  return *(const gdbl*)((const gchar*)(uintptr_t)tab + (unsigned)idx * 8u);
+#else
+  return tab[idx];
+#endif
+}
+
+// ip12_im12 with device gathers
+__device__ inline void ip12_im12_dev(const double* tab, double a, double& ip, double& im) {
+  if (a > AMAX || a < AMIN) {
+    ip = ip12(tab, a);
+    im = im12(tab, a);
+    return;
+  }
+  const int i = node(a);
+  const int o = i * NC;
+  const double t = (a - (AMIN + i * H)) * INVH;
+  const double t2 = t * t, t3 = t2 * t;
+  const double h00 = 2 * t3 - 3 * t2 + 1, h10 = (t3 - 2 * t2 + t) * H, h01 = -2 * t3 + 3 * t2,
+               h11 = (t3 - t2) * H;
+  const double m0 = tabd(tab, o), p0 = tabd(tab, o + 1), d0 = tabd(tab, o + 3);
+  const double m1 = tabd(tab, o + NC), p1 = tabd(tab, o + NC + 1), d1 = tabd(tab, o + NC + 3);
+  ip = h00 * p0 + h10 * (0.5 * m0) + h01 * p1 + h11 * (0.5 * m1);
+  im = h00 * m0 + h10 * d0 + h01 * m1 + h11 * d1;
+}
+
+// im12_d1_d2 with device gathers
+__device__ inline void im12_d1_d2_dev(const double* tab, double a, double& im, double& d1, double& d2) {
+  if (a > AMAX || a < AMIN) {
+    im = im12(tab, a);
+    d1 = dim12(tab, a);
+    d2 = d2im12(tab, a);
+    return;
+  }
+  const int i = node(a);
+  const int o = i * NC;
+  const double t = (a - (AMIN + i * H)) * INVH;
+  const double t2 = t * t, t3 = t2 * t;
+  const double h00 = 2 * t3 - 3 * t2 + 1, h10 = (t3 - 2 * t2 + t) * H, h01 = -2 * t3 + 3 * t2,
+               h11 = (t3 - t2) * H;
+  const double a0 = tabd(tab, o), a3 = tabd(tab, o + 3), a4 = tabd(tab, o + 4), a5 = tabd(tab, o + 5);
+  const double b0 = tabd(tab, o + NC), b3 = tabd(tab, o + NC + 3), b4 = tabd(tab, o + NC + 4),
+               b5 = tabd(tab, o + NC + 5);
+  im = h00 * a0 + h10 * a3 + h01 * b0 + h11 * b3;
+  d1 = h00 * a3 + h10 * a4 + h01 * b3 + h11 * b4;
+  d2 = h00 * a4 + h10 * a5 + h01 * b4 + h11 * b5;
+}
+
+// 1/d from the hardware reciprocal and one Newton-Raphson correction
+__device__ inline double rcp_nr(double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r = __builtin_amdgcn_rcp(d);
+  return fma(fma(-d, r, 1.0), r, r);
+#else
+  return 1.0 / d;
+#endif
+}
+
+// alpha_from_ip12_from with device gathers (same iteration, safeguard and
+// stopping rule; the step's division is the hardware reciprocal refined once)
+__device__ inline double alpha_from_ip12_dev(const double* tab, double y, double a_start) {
+  const double huge = 1.7976931348623157e308;
+  double alpha = a_start;
+  double lo = -huge, hi = huge;
+  for (int it = 0; it < 100; ++it) {
+    double ip, im;
+    ip12_im12_dev(tab, alpha, ip, im);
+    const double f = ip - y;
+    if (f > 0.0) hi = fmin(hi, alpha);
+    else lo = fmax(lo, alpha);
+    const double step = f * rcp_nr(0.5 * im);
     if (fabs(step) < 1.0e-14 * fmax(1.0, fabs(alpha))) break;
     alpha = alpha - step;
     if (alpha <= lo || alpha >= hi) {

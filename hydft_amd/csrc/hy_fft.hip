@@ -8,6 +8,7 @@
 #include <mutex>
 
 #include "hy_internal.h"
+#include "hy_regfft.h"
 
 namespace hy {
 
@@ -201,83 +202,160 @@ __device__ inline cpx<T> load_any(const void* p, size_t i, int is64) {
   return cvt<T>(reinterpret_cast<const c32*>(p)[i]);
 }
 
-template <class T, int L, int NA, int NT>
-__global__ __launch_bounds__(NT) void k_pencil_inv(PencilArgs a, const cpx<T>* __restrict__ twg) {
-  __shared__ cpx<T> buf[L * NA];
-  __shared__ cpx<T> tw[L];
-  const int tid = threadIdx.x;
-  const int a0 = blockIdx.x * NA, b = blockIdx.y;
-  const PencilField& f = a.f[blockIdx.z];
-  for (int j = tid; j < L; j += NT) tw[j] = twg[j];
+// Block shape of the pencil kernels: NA contiguous columns (128 bytes of one
+// row per quarter wave) times TPC threads per column; columns of length
+// <= 32 take one thread each, 64 per block.
+template <class T, int L>
+struct PencilShape {
+  static constexpr ColPlan P = col_plan(L, (int)sizeof(cpx<T>));
+  static constexpr int L1 = P.L1, L2 = P.L2, TPC = P.TPC;
+  static constexpr int NA = TPC == 1 ? 64 : (sizeof(T) == 4 ? 16 : 8);
+  static constexpr int NT = NA * TPC;
+  static constexpr int J1 = L2 / TPC;  // step-1 DFTs (length L1) per thread
+  static constexpr int J2 = L1 / TPC;  // step-2 DFTs (length L2) per thread
+};
+
+// stored index of position p along the axis, or -1 outside the kept range
+__device__ inline int kept_index(int p, int h, int tailoff) {
+  return p < h ? p : (p >= h + tailoff ? p - tailoff : -1);
+}
+
+// Inverse pass: in (A, K, B) stored spectrum (kept entries only) -> out (A, L, B).
+template <class T, int L>
+__global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_inv(PencilArgs a, const cpx<T>* __restrict__ twg) {
+  using S = PencilShape<T, L>;
+  constexpr int L1 = S::L1, L2 = S::L2, TPC = S::TPC, NA = S::NA;
+  const int al = threadIdx.x % NA, g = threadIdx.x / NA;
+  // 1-D grid of ntile * B * nf workgroups in XCD-aware order
+  const int ntile = (a.A + NA - 1) / NA;
+  const unsigned w = xcd_work_index(blockIdx.x, gridDim.x);
+  const int tile = (int)(w % (unsigned)ntile);
+  const int b = (int)((w / (unsigned)ntile) % (unsigned)a.B);
+  const int fi = (int)(w / ((unsigned)ntile * (unsigned)a.B));
+  const int ga = tile * NA + al;
+  const PencilField& f = a.f[fi];
+  const bool live = ga < a.A;
   const int tailoff = L - a.K;
-  for (int e = tid; e < L * NA; e += NT) {
-    const int p = e / NA, al = e % NA, ga = a0 + al;
-    int t = -1;
-    if (p < a.h) t = p;
-    else if (p >= a.h + tailoff) t = p - tailoff;
+  auto load = [&](int p) -> cpx<T> {
+    const int t = kept_index(p, a.h, tailoff);
     cpx<T> v = mk<T>(0, 0);
-    if (t >= 0 && ga < a.A) {
+    if (t >= 0 && live) {
       v = load_any<T>(f.in, ((size_t)b * a.K + t) * a.A + ga, f.in_f64);
       if (f.k0_offset != 0.0 && t == 0 && ga == 0 && b == 0) v.x = (T)((double)v.x + f.k0_offset);
       v = apply_mul<T>(v, f.mul, p, ga, a);
     }
-    buf[e] = v;
-  }
-  __syncthreads();
-  fft_lds<T, L, NA, NT, true>(buf, tw, tid);
+    return v;
+  };
   cpx<T>* out = reinterpret_cast<cpx<T>*>(f.out);
-  for (int e = tid; e < L * NA; e += NT) {
-    const int p = e / NA, al = e % NA, ga = a0 + al;
-    if (ga < a.A) out[((size_t)b * L + p) * a.A + ga] = buf[e];
+  if constexpr (TPC == 1) {
+    cpx<T> v[L];
+#pragma unroll
+    for (int p = 0; p < L; ++p) v[p] = load(p);
+    rdft<T, L, true>(v);
+    if (live) {
+#pragma unroll
+      for (int p = 0; p < L; ++p) out[((size_t)b * L + p) * a.A + ga] = v[p];
+    }
+  } else {
+    __shared__ cpx<T> lds[L * NA];
+#pragma unroll
+    for (int j = 0; j < S::J1; ++j) {
+      const int n2 = g + TPC * j;
+      cpx<T> v[L1];
+#pragma unroll
+      for (int n1 = 0; n1 < L1; ++n1) v[n1] = load(L2 * n1 + n2);
+      rdft<T, L1, true>(v);
+#pragma unroll
+      for (int k1 = 0; k1 < L1; ++k1) {
+        cpx<T> w = v[k1];
+        if (k1 > 0) w = cmulc(w, twg[(n2 * k1) % L]);
+        lds[(k1 * L2 + n2) * NA + al] = w;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < S::J2; ++j) {
+      const int k1 = g + TPC * j;
+      cpx<T> w[L2];
+#pragma unroll
+      for (int n2 = 0; n2 < L2; ++n2) w[n2] = lds[(k1 * L2 + n2) * NA + al];
+      rdft<T, L2, true>(w);
+      if (live) {
+#pragma unroll
+        for (int k2 = 0; k2 < L2; ++k2) out[((size_t)b * L + k1 + L1 * k2) * a.A + ga] = w[k2];
+      }
+    }
   }
 }
 
-template <class T, int L, int NA, int NT>
-__global__ __launch_bounds__(NT) void k_pencil_fwd(PencilArgs a, const cpx<T>* __restrict__ twg) {
-  __shared__ cpx<T> buf[L * NA];
-  __shared__ cpx<T> tw[L];
-  const int tid = threadIdx.x;
-  const int a0 = blockIdx.x * NA, b = blockIdx.y;
-  const PencilField& f = a.f[blockIdx.z];
-  for (int j = tid; j < L; j += NT) tw[j] = twg[j];
-  for (int e = tid; e < L * NA; e += NT) {
-    const int p = e / NA, al = e % NA, ga = a0 + al;
-    buf[e] = ga < a.A ? load_any<T>(f.in, ((size_t)b * L + p) * a.A + ga, f.in_f64) : mk<T>(0, 0);
-  }
-  __syncthreads();
-  fft_lds<T, L, NA, NT, false>(buf, tw, tid);
-  cpx<T>* out = reinterpret_cast<cpx<T>*>(f.out);
+// Forward pass: in (A, L, B) -> out (A, K, B) kept entries, scaled by 1/L.
+template <class T, int L>
+__global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_fwd(PencilArgs a, const cpx<T>* __restrict__ twg) {
+  using S = PencilShape<T, L>;
+  constexpr int L1 = S::L1, L2 = S::L2, TPC = S::TPC, NA = S::NA;
+  const int al = threadIdx.x % NA, g = threadIdx.x / NA;
+  // 1-D grid of ntile * B * nf workgroups in XCD-aware order
+  const int ntile = (a.A + NA - 1) / NA;
+  const unsigned w = xcd_work_index(blockIdx.x, gridDim.x);
+  const int tile = (int)(w % (unsigned)ntile);
+  const int b = (int)((w / (unsigned)ntile) % (unsigned)a.B);
+  const int fi = (int)(w / ((unsigned)ntile * (unsigned)a.B));
+  const int ga = tile * NA + al;
+  const PencilField& f = a.f[fi];
+  const bool live = ga < a.A;
   const int tailoff = L - a.K;
   const T sc = (T)a.scale;
-  for (int e = tid; e < a.K * NA; e += NT) {
-    const int t = e / NA, al = e % NA, ga = a0 + al;
-    if (ga >= a.A) continue;
-    const int p = t < a.h ? t : t + tailoff;
-    cpx<T> v = scal(buf[p * NA + al], sc);
-    v = apply_mul<T>(v, f.mul, p, ga, a);
-    out[((size_t)b * a.K + t) * a.A + ga] = v;
+  auto load = [&](int p) -> cpx<T> {
+    return live ? load_any<T>(f.in, ((size_t)b * L + p) * a.A + ga, f.in_f64) : mk<T>(0, 0);
+  };
+  cpx<T>* out = reinterpret_cast<cpx<T>*>(f.out);
+  auto store = [&](int p, cpx<T> v) {
+    const int t = kept_index(p, a.h, tailoff);
+    if (t >= 0 && live) out[((size_t)b * a.K + t) * a.A + ga] = apply_mul<T>(scal(v, sc), f.mul, p, ga, a);
+  };
+  if constexpr (TPC == 1) {
+    cpx<T> v[L];
+#pragma unroll
+    for (int p = 0; p < L; ++p) v[p] = load(p);
+    rdft<T, L, false>(v);
+#pragma unroll
+    for (int p = 0; p < L; ++p) store(p, v[p]);
+  } else {
+    __shared__ cpx<T> lds[L * NA];
+#pragma unroll
+    for (int j = 0; j < S::J1; ++j) {
+      const int n2 = g + TPC * j;
+      cpx<T> v[L1];
+#pragma unroll
+      for (int n1 = 0; n1 < L1; ++n1) v[n1] = load(L2 * n1 + n2);
+      rdft<T, L1, false>(v);
+#pragma unroll
+      for (int k1 = 0; k1 < L1; ++k1) {
+        cpx<T> w = v[k1];
+        if (k1 > 0) w = cmul(w, twg[(n2 * k1) % L]);
+        lds[(k1 * L2 + n2) * NA + al] = w;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < S::J2; ++j) {
+      const int k1 = g + TPC * j;
+      cpx<T> w[L2];
+#pragma unroll
+      for (int n2 = 0; n2 < L2; ++n2) w[n2] = lds[(k1 * L2 + n2) * NA + al];
+      rdft<T, L2, false>(w);
+#pragma unroll
+      for (int k2 = 0; k2 < L2; ++k2) store(k1 + L1 * k2, w[k2]);
+    }
   }
-}
-
-// Tile of NA pencils per block, at most 16 KB of LDS (1024 c64 or 2048 c32).
-template <class T>
-constexpr int pencil_na(int L) {
-  int na = (sizeof(T) == 16 ? 1024 : 2048) / L;
-  return na < 4 ? 4 : (na > 64 ? 64 : na);
-}
-template <class T>
-constexpr int pencil_nt(int L) {
-  int nt = L * pencil_na<T>(L) / 4;
-  return nt < 64 ? 64 : (nt > 256 ? 256 : nt);
 }
 
 template <class T, int L>
 static void launch_pencil(const PencilArgs& a, bool inverse, int nf, const cpx<T>* tw, hipStream_t s) {
-  constexpr int NA = pencil_na<cpx<T>>(L);
-  constexpr int NT = pencil_nt<cpx<T>>(L);
-  dim3 grid((a.A + NA - 1) / NA, a.B, nf);
-  if (inverse) hipLaunchKernelGGL((k_pencil_inv<T, L, NA, NT>), grid, dim3(NT), 0, s, a, tw);
-  else hipLaunchKernelGGL((k_pencil_fwd<T, L, NA, NT>), grid, dim3(NT), 0, s, a, tw);
+  using S = PencilShape<T, L>;
+  dim3 grid((unsigned)((a.A + S::NA - 1) / S::NA) * (unsigned)a.B * (unsigned)nf);
+  if (inverse) hipLaunchKernelGGL((k_pencil_inv<T, L>), grid, dim3(S::NT), 0, s, a, tw);
+  else hipLaunchKernelGGL((k_pencil_fwd<T, L>), grid, dim3(S::NT), 0, s, a, tw);
 }
 
 template <class T>

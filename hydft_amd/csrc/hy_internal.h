@@ -62,6 +62,18 @@ __host__ __device__ inline cpx<T> mul_i(cpx<T> a) { return mk<T>(-a.y, a.x); }  
 template <class TO, class TI>
 __host__ __device__ inline cpx<TO> cvt(cpx<TI> a) { return mk<TO>((TO)a.x, (TO)a.y); }
 
+// ------------------------------------------------------ XCD-aware workgroup order
+// Workgroups of a launch are dealt round-robin over the chip's 8 XCDs
+// (workgroup i runs on XCD i % 8), each with its own L2.  Kernels whose
+// neighbouring work items share cache lines (column tiles of one row,
+// adjacent x-lines) take their work index from this map, so that
+// consecutive work indices run on one XCD at about the same time.
+constexpr unsigned NXCD = 8;
+__device__ inline unsigned xcd_work_index(unsigned i, unsigned n) {
+  const unsigned q = n / NXCD;
+  return i < q * NXCD ? (i % NXCD) * q + i / NXCD : i;
+}
+
 // ------------------------------------------------------ radix plans (2^a 3^b)
 // Stage radices: as many 4s as possible, then one 2, then 3s.
 __host__ __device__ constexpr int n4_of(int L) { int n = 0; while (L % 4 == 0) { L /= 4; ++n; } return n; }

@@ -130,6 +130,59 @@ __global__ void k_ctl(Ctl* c, int op, int s, double tau, int evolve, double tend
   }
 }
 
+// ------------------------------------------------------------ global memory
+// (Device compilation; a host-side build of this file, such as a CPU
+// emulation, uses plain pointers.)
+// Loads and stores through the global address space with a 32-bit byte
+// offset from a uniform base.  The x-line kernels take their buffer pointers
+// from an argument block in memory, which the compiler cannot place in an
+// address space; this form gives scalar-base global accesses instead of
+// 64-bit flat addresses per lane.  Every buffer addressed this way is below
+// 4 GiB.
+template <class V>
+__device__ inline V gld(const void* base, size_t elem) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(1))) const char gchar;
+  typedef __attribute__((address_space(1))) const V gV;
+  return *(const gV*)((const gchar*)(uintptr_t)base + (unsigned)(elem * sizeof(V)));
+#else
+  return reinterpret_cast<const V*>(base)[elem];
+#endif
+}
+template <class V>
+__device__ inline void gst(void* base, size_t elem, V v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(1))) char gchar;
+  typedef __attribute__((address_space(1))) V gV;
+  *(gV*)((gchar*)(uintptr_t)base + (unsigned)(elem * sizeof(V))) = v;
+#else
+  reinterpret_cast<V*>(base)[elem] = v;
+#endif
+}
+// complex elements move as two-component vectors (one 8- or 16-byte access)
+template <class T>
+__device__ inline cpx<T> gldc(const void* base, size_t elem) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  const v2 v = gld<v2>(base, elem);
+  return mk<T>(v.x, v.y);
+#else
+  return reinterpret_cast<const cpx<T>*>(base)[elem];
+#endif
+}
+template <class T>
+__device__ inline void gstc(void* base, size_t elem, cpx<T> c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  v2 v;
+  v.x = c.x;
+  v.y = c.y;
+  gst<v2>(base, elem, v);
+#else
+  reinterpret_cast<cpx<T>*>(base)[elem] = c;
+#endif
+}
+
 // ------------------------------------------------------------ x-line inputs
 struct XIn {
   const void* p;   // (Kx, nlines) spectral x-lines
@@ -141,75 +194,156 @@ enum XMul { XM_NONE = 0, XM_IK = 1, XM_NEGK2 = 2 };
 
 template <class T>
 __device__ inline cpx<T> xin_load(const XIn& in, size_t line, int Kx, int kk, const double* kx) {
-  cpx<T> v = in.is64 ? cvt<T>(reinterpret_cast<const c64*>(in.p)[line * Kx + kk])
-                     : reinterpret_cast<const cpx<T>*>(in.p)[line * Kx + kk];
+  const size_t e = line * Kx + kk;
+  cpx<T> v = in.is64 ? cvt<T>(gldc<double>(in.p, e)) : gldc<T>(in.p, e);
   if (in.k0off != 0.0 && kk == 0 && line == 0) v.x = (T)((double)v.x + in.k0off);
   if (in.mul == XM_IK) {
-    double k = kx[kk];
-    v = mk<T>((T)(-k * (double)v.y), (T)(k * (double)v.x));
+    const T k = (T)kx[kk];
+    v = mk<T>(-k * v.y, k * v.x);
   } else if (in.mul == XM_NEGK2) {
-    double k2 = kx[kk] * kx[kk];
-    v = mk<T>((T)(-k2 * (double)v.x), (T)(-k2 * (double)v.y));
+    const T k2 = (T)(kx[kk] * kx[kk]);
+    v = mk<T>(-k2 * v.x, -k2 * v.y);
   }
   return v;
 }
 
-// Inverse x transform of two spectral x-lines packed as a + i b; afterwards
-// buf[x] = a(x) + i b(x).  DC and Nyquist imaginary parts are dropped, as a
-// c2r transform does.
+// ---- x-line transforms on one 64-lane wave
+// An NX-point line is transformed by a Stockham sequence whose first stage
+// reads its operands from the caller (global loads or the lane's registers)
+// and whose last stage hands its results back in registers; only the middle
+// stages exchange data through two LDS buffers (one barrier per stage).  The
+// inverse transforms use the radix order radix_of(NX, s) and the forward ones
+// the reversed order, so the points a lane holds after an inverse transform
+// are exactly the operands of its first forward stage: lane `lane` owns the
+// XQ<NX>::Q points x = xown<NX>(lane, q).
+template <int L, bool REV>
+__host__ __device__ constexpr int prad(int s) { return REV ? radix_of(L, nstages_of(L) - 1 - s) : radix_of(L, s); }
+template <int L, bool REV>
+__host__ __device__ constexpr int pns(int s) { int p = 1; for (int i = 0; i < s; ++i) p *= prad<L, REV>(i); return p; }
+
+// LDS slot of element e of a line buffer or twiddle table: bits 4-5 of e are
+// XORed into bits 0-1 and 2-3.  The strided Stockham stores (stride 4 in the
+// first stage, groups of 4 in the second) and the twiddle reads then hit 16
+// different bank pairs per 16 lanes.  The map permutes every aligned block
+// of 16 elements (and [16, 24)), so it is a bijection on [0, NX) for every
+// supported NX.
+__device__ constexpr int swz(int e) { return e ^ (((e >> 4) & 3) * 5); }
+
+template <int NX>
+struct XQ {
+  static constexpr int RL = radix_of(NX, nstages_of(NX) - 1);  // last inverse radix
+  static constexpr int NBL = NX / RL;
+  static constexpr int PERL = (NBL + 63) / 64;
+  static constexpr int Q = PERL * RL;  // points per lane
+};
+// point x of the lane's q-th value, or -1 (lane idle for that slot)
+template <int NX>
+__device__ inline int xown(int lane, int q) {
+  const int j = lane + 64 * (q / XQ<NX>::RL);
+  return j < XQ<NX>::NBL ? j + (q % XQ<NX>::RL) * XQ<NX>::NBL : -1;
+}
+
+template <class T, int L, bool INV, bool REV, int S, class In, class Out>
+__device__ inline void xstage(const cpx<T>* src, cpx<T>* dst, const cpx<T>* tw, int lane, In& in, Out& out) {
+  constexpr int NST = nstages_of(L);
+  constexpr int R = prad<L, REV>(S);
+  constexpr int NS = pns<L, REV>(S);
+  constexpr int NB = L / R;
+  constexpr int PER = (NB + 63) / 64;
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int j = lane + 64 * b;
+    if (NB % 64 == 0 || j < NB) {
+      cpx<T> v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if constexpr (S == 0) v[r] = in(b, r, j + r * NB);
+        else v[r] = src[swz(j + r * NB)];
+      }
+      const int k = j % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const cpx<T> w = tw[swz(k * r * (L / (NS * R)))];
+          v[r] = INV ? cmulc(v[r], w) : cmul(v[r], w);
+        }
+      }
+      dftR<T, INV, R>(v);
+      if constexpr (S == NST - 1) {  // NS == NB here: outputs sit where the inputs were
+#pragma unroll
+        for (int r = 0; r < R; ++r) out(b, r, j + r * NB, v[r]);
+      } else {
+        const int base = (j / NS) * NS * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[swz(base + r * NS)] = v[r];
+      }
+    }
+  }
+}
+
+// stage S reads buf[(S+1)%2] (S > 0) and writes buf[S%2]
+template <class T, int L, bool INV, bool REV, int S = 0, class In, class Out>
+__device__ inline void xfft(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, int lane, In& in, Out& out) {
+  if constexpr (S < nstages_of(L)) {
+    xstage<T, L, INV, REV, S>(S % 2 ? b0 : b1, S % 2 ? b1 : b0, tw, lane, in, out);
+    if constexpr (S + 1 < nstages_of(L)) __syncthreads();
+    xfft<T, L, INV, REV, S + 1>(b0, b1, tw, lane, in, out);
+  }
+}
+
+// Inverse x transform of two spectral x-lines packed as a + i b (a c2r pair:
+// DC and Nyquist imaginary parts dropped); the lane's points of a(x) and
+// b(x) are returned in qa[q], qb[q] (qb may be null).
 template <class T, int NX>
-__device__ inline void x_inv_pair(cpx<T>* buf, const cpx<T>* tw, const XIn* A, const XIn* B,
-                                  size_t line, int Kx, const double* kx, int lane) {
-  __syncthreads();
-  for (int kk = lane; kk <= NX / 2; kk += 64) {
+__device__ inline void x_inv_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, const XIn* A, const XIn* B,
+                                  size_t line, int Kx, const double* kx, int lane, T* qa, T* qb) {
+#pragma unroll
+  for (int q = 0; q < XQ<NX>::Q; ++q) {  // slots of idle lanes stay defined
+    qa[q] = 0;
+    if (qb) qb[q] = 0;
+  }
+  __syncthreads();  // the previous transform's readers of b0/b1 are done
+  auto in = [&](int, int, int e) -> cpx<T> {
+    const bool hi = 2 * e > NX;
+    const int kk = hi ? NX - e : e;
     cpx<T> a = mk<T>(0, 0), b = mk<T>(0, 0);
     if (kk < Kx) {
       if (A) a = xin_load<T>(*A, line, Kx, kk, kx);
       if (B) b = xin_load<T>(*B, line, Kx, kk, kx);
     }
     if (kk == 0 || 2 * kk == NX) { a.y = 0; b.y = 0; }
-    buf[kk] = mk<T>(a.x - b.y, a.y + b.x);
-    if (kk > 0 && 2 * kk < NX) buf[NX - kk] = mk<T>(a.x + b.y, -a.y + b.x);  // conj(a) + i conj(b)
-  }
-  __syncthreads();
-  fft_lds<T, NX, 1, 64, true>(buf, tw, lane);
+    return hi ? mk<T>(a.x + b.y, -a.y + b.x)   // conj(a) + i conj(b)
+              : mk<T>(a.x - b.y, a.y + b.x);   // a + i b
+  };
+  auto out = [&](int b, int r, int, cpx<T> v) {
+    const int q = b * XQ<NX>::RL + r;
+    qa[q] = v.x;
+    if (qb) qb[q] = v.y;
+  };
+  xfft<T, NX, true, false>(b0, b1, tw, lane, in, out);
 }
 
 // Forward x transform of two real lines packed as va + i vb, where
-// val(p) = (va, vb) at the lane's p-th point x = lane + 64 p; stores the
-// first Kx coefficients of each, scaled by 1/NX, to outA / outB (compute type).
-template <class T, int NX, int PTS, class F>
-__device__ inline void x_fwd_pair(cpx<T>* buf, const cpx<T>* tw, F&& val, void* outA, void* outB,
+// val(q) = (va, vb) at the lane's q-th point; stores the first Kx
+// coefficients of each, scaled by 1/NX, to outA / outB (compute type).
+template <class T, int NX, class F>
+__device__ inline void x_fwd_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, F&& val, void* outA, void* outB,
                                   size_t line, int Kx, int lane) {
+  constexpr int NST = nstages_of(NX);
+  cpx<T>* res = (NST - 1) % 2 ? b1 : b0;  // where the last stage stores
   __syncthreads();
-#pragma unroll
-  for (int p = 0; p < PTS; ++p) {
-    int x = lane + 64 * p;
-    if (x < NX) buf[x] = val(p);
-  }
+  auto in = [&](int b, int r, int) -> cpx<T> { return val(b * XQ<NX>::RL + r); };
+  auto out = [&](int, int, int e, cpx<T> v) { res[swz(e)] = v; };
+  xfft<T, NX, false, true>(b0, b1, tw, lane, in, out);
   __syncthreads();
-  fft_lds<T, NX, 1, 64, false>(buf, tw, lane);
   const T sc = (T)(0.5 / NX);
-  cpx<T>* oA = reinterpret_cast<cpx<T>*>(outA);
-  cpx<T>* oB = reinterpret_cast<cpx<T>*>(outB);
   for (int k = lane; k < Kx; k += 64) {
-    cpx<T> Z = buf[k], Zc = conjc(buf[(NX - k) % NX]);
-    oA[line * Kx + k] = scal(Z + Zc, sc);
-    if (oB) {
+    cpx<T> Z = res[swz(k)], Zc = conjc(res[swz((NX - k) % NX)]);
+    gstc<T>(outA, line * Kx + k, scal(Z + Zc, sc));
+    if (outB) {
       cpx<T> D = Z - Zc;
-      oB[line * Kx + k] = mk<T>(D.y * sc, -D.x * sc);
+      gstc<T>(outB, line * Kx + k, mk<T>(D.y * sc, -D.x * sc));
     }
-  }
-}
-
-template <class T, int NX, int PTS>
-__device__ inline void x_get(const cpx<T>* buf, double* a, double* b, int lane) {
-#pragma unroll
-  for (int p = 0; p < PTS; ++p) {
-    int x = lane + 64 * p;
-    cpx<T> v = x < NX ? buf[x] : mk<T>(0, 0);
-    if (a) a[p] = (double)v.x;
-    if (b) b[p] = (double)v.y;
   }
 }
 
@@ -221,6 +355,32 @@ __device__ inline double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   return v;
 }
+
+// ---- x-line layout of the x-line kernels: one line per 64-lane wave, the
+// Stockham transforms above.  The kernels take the layout as a policy: the
+// lane's line (line_in_wave) and position g in it, its points x(g, q) for
+// q < PTS, the pair transforms and per-line reductions.
+template <class T, int NX>
+struct XWave {
+  static constexpr int LPW = 1, TPC = 64, PTS = XQ<NX>::Q;
+  struct Smem {
+    cpx<T> b0[NX], b1[NX], tw[NX];
+  };
+  __device__ static void init(Smem& s, const cpx<T>* twg, int lane) {
+    for (int j = lane; j < NX; j += 64) s.tw[swz(j)] = twg[j];
+  }
+  __device__ static int xpos(int g, int q) { return xown<NX>(g, q); }
+  __device__ static void inv_pair(Smem& s, const XIn* A, const XIn* B, size_t line, bool, int Kx,
+                                  const double* kx, int g, int, T* qa, T* qb) {
+    x_inv_pair<T, NX>(s.b0, s.b1, s.tw, A, B, line, Kx, kx, g, qa, qb);
+  }
+  template <class F>
+  __device__ static void fwd_pair(Smem& s, F&& val, void* oA, void* oB, size_t line, bool, int Kx, int g, int) {
+    x_fwd_pair<T, NX>(s.b0, s.b1, s.tw, val, oA, oB, line, Kx, g);
+  }
+  __device__ static double line_sum(double v) { return wave_sum(v); }
+  __device__ static double line_max(double v) { return wave_max(v); }
+};
 
 // ---------------------------------------------------------- physics args
 enum OutA { OA_MU = 0, OA_U1, OA_U2, OA_U3, OA_F11, OA_F12, OA_F13, OA_F22, OA_F23, OA_F33 };
@@ -257,169 +417,172 @@ struct PhysArgs {
   void* dst[10];
 };
 
-template <class T, int NX>
+// Common preamble of the x-line kernels: the lane's line and position in it.
+#define HY_XLINE_PREAMBLE                                                              \
+  constexpr int PTS = X::PTS;                                                          \
+  __shared__ typename X::Smem sm;                                                      \
+  const PhysArgs& a = *pa;                                                             \
+  const int lane = threadIdx.x;                                                        \
+  const int lw = lane / X::TPC, g = lane % X::TPC;                                     \
+  const size_t line = (size_t)xcd_work_index(blockIdx.x, gridDim.x) * X::LPW + lw;     \
+  const bool lok = line < (size_t)a.ny * a.nz;                                         \
+  const int nd = a.nd;                                                                 \
+  X::init(sm, twg, lane);
+
+template <class T, int NX, class X>
 __global__ __launch_bounds__(64) void k_phaseA(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
-  constexpr int PTS = NX >= 64 ? NX / 64 : 1;
-  __shared__ cpx<T> buf[NX];
-  __shared__ cpx<T> tw[NX];
-  const PhysArgs& a = *pa;
-  const int lane = threadIdx.x;
-  const size_t line = blockIdx.x;
-  const int nd = a.nd;
-  for (int j = lane; j < NX; j += 64) tw[j] = twg[j];
-  double dn[PTS], m0[PTS], m1[PTS], m2[PTS];
-  x_inv_pair<T, NX>(buf, tw, &a.n, &a.m[0], line, a.Kx, a.kx, lane);
-  x_get<T, NX, PTS>(buf, dn, m0, lane);
-  if (nd >= 2) {
-    x_inv_pair<T, NX>(buf, tw, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, line, a.Kx, a.kx, lane);
-    x_get<T, NX, PTS>(buf, m1, m2, lane);
-  } else {
+  HY_XLINE_PREAMBLE
+  // per-point values are held in the transform type T: in mixed precision
+  // they come out of (and go back into) FP32 transforms; the pointwise
+  // functional below is evaluated in FP64
+  // Transform pairs ordered so that few fields are live at once: the
+  // functional needs n - n0, |grad n|^2 and lap n; the outputs need mom.
+  T dn[PTS], m0[PTS], g2[PTS], lap[PTS];
 #pragma unroll
-    for (int p = 0; p < PTS; ++p) { m1[p] = 0.0; m2[p] = 0.0; }
-  }
-  double mu[PTS], u0[PTS], u1[PTS], u2[PTS];
-  double gx[PTS], gy[PTS], gz[PTS], lap[PTS];
-#pragma unroll
-  for (int p = 0; p < PTS; ++p) { gx[p] = 0.0; gy[p] = 0.0; gz[p] = 0.0; lap[p] = 0.0; }
+  for (int p = 0; p < PTS; ++p) { m0[p] = 0; g2[p] = 0; lap[p] = 0; }
   if (a.grad) {
-    x_inv_pair<T, NX>(buf, tw, &a.gx, nd >= 2 ? &a.gy : nullptr, line, a.Kx, a.kx, lane);
-    x_get<T, NX, PTS>(buf, gx, gy, lane);
-    x_inv_pair<T, NX>(buf, tw, nd >= 3 ? &a.gz : nullptr, &a.lap, line, a.Kx, a.kx, lane);
-    x_get<T, NX, PTS>(buf, gz, lap, lane);
+    X::inv_pair(sm, &a.n, &a.lap, line, lok, a.Kx, a.kx, g, lw, dn, lap);
+    T gx[PTS], gy[PTS];
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) gy[p] = 0;
+    X::inv_pair(sm, &a.gx, nd >= 2 ? &a.gy : nullptr, line, lok, a.Kx, a.kx, g, lw, gx, gy);
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) g2[p] = gx[p] * gx[p] + gy[p] * gy[p];
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) gx[p] = 0;
+    X::inv_pair(sm, nd >= 3 ? &a.gz : nullptr, &a.m[0], line, lok, a.Kx, a.kx, g, lw, gx, m0);
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) g2[p] += gx[p] * gx[p];
+  } else {
+    X::inv_pair(sm, &a.n, &a.m[0], line, lok, a.Kx, a.kx, g, lw, dn, m0);
   }
+  // pointwise functional in FP64
+  T mu[PTS], inv[PTS];  // mu_loc, 1 / (m n)
   bool bad = false;
 #pragma unroll
   for (int p = 0; p < PTS; ++p) {
-    const bool live = (lane + 64 * p) < NX;
-    const double n = a.n0 + dn[p];
+    const bool live = lok && X::xpos(g, p) >= 0;
+    const double n = a.n0 + (double)dn[p];
     if (live && !(n > 0.0)) bad = true;
-    const double inv = 1.0 / (a.mass * n);
-    u0[p] = m0[p] * inv; u1[p] = m1[p] * inv; u2[p] = m2[p] * inv;
+    inv[p] = (T)(1.0 / (a.mass * n));
     // mu_loc with its k = 0 constants (ln n0 / beta, alpha0 / beta) dropped:
     // only grad(mu) enters the momentum equation (model.f90:99).
-    double v = 0.0;
-    if (a.ideal) v += log1p(dn[p] / a.n0) / a.beta;  // fe_ideal_classical.f90:40
+    double v = a.ideal ? log1p((double)dn[p] / a.n0) / a.beta : 0.0;  // fe_ideal_classical.f90:40
     if (a.tf || a.grad) {
       const double al = live && n > 0.0
-                            ? fd::alpha_from_ip12_from(a.fdtab, n / a.cn, a.alpha0 + a.dadn * dn[p])
+                            ? fd::alpha_from_ip12_dev(a.fdtab, n / a.cn, a.alpha0 + a.dadn * (double)dn[p])
                             : a.alpha0;
       if (a.tf) v += (al - a.alpha0) / a.beta;  // fe_kinetic_tfk.f90:80-81
       if (a.grad) {                             // fe_kinetic_tfk.f90:130-137
         double im, d1, d2;
-        fd::im12_d1_d2(a.fdtab, al, im, d1, d2);
-        const double xi = d1 / (im * im);
-        const double dxi = (d2 * im - 2.0 * d1 * d1) / (im * im * im);
-        const double dxidn = dxi / (0.5 * a.cn * im);
-        const double g2 = gx[p] * gx[p] + gy[p] * gy[p] + gz[p] * gz[p];
-        v -= a.cg * (dxidn * g2 + 2.0 * xi * lap[p]);
+        fd::im12_d1_d2_dev(a.fdtab, al, im, d1, d2);
+        const double rim = 1.0 / im;
+        const double xi = d1 * rim * rim;
+        const double dxi = (d2 * im - 2.0 * d1 * d1) * (rim * rim * rim);
+        const double dxidn = dxi * rim * (2.0 / a.cn);
+        v -= a.cg * (dxidn * (double)g2[p] + 2.0 * xi * (double)lap[p]);
       }
     }
-    mu[p] = v;
+    mu[p] = (T)v;
   }
   if (bad) a.ctl->nonpos = 1;
-  auto val = [&](int c, int p) -> double {
+  T m1[PTS], m2[PTS];
+#pragma unroll
+  for (int p = 0; p < PTS; ++p) { m1[p] = 0; m2[p] = 0; }
+  if (nd >= 2) X::inv_pair(sm, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, line, lok, a.Kx, a.kx, g, lw, m1, m2);
+  // outputs, one uniform switch per output (not per point)
+  auto fill = [&](int c, T* v) {
+#pragma unroll
+    for (int p = 0; p < PTS; ++p) v[p] = 0;
+    // every case names its arrays directly: selecting an array through a
+    // run-time pointer would move the arrays to scratch memory
+#define HY_FILL(EXPR) for (int p = 0; p < PTS; ++p) v[p] = (T)(EXPR)
     switch (c) {
-      case OA_MU: return mu[p];
-      case OA_U1: return u0[p];
-      case OA_U2: return u1[p];
-      case OA_U3: return u2[p];
-      case OA_F11: return m0[p] * u0[p];
-      case OA_F12: return m0[p] * u1[p];
-      case OA_F13: return m0[p] * u2[p];
-      case OA_F22: return m1[p] * u1[p];
-      case OA_F23: return m1[p] * u2[p];
-      case OA_F33: return m2[p] * u2[p];
-      default: return 0.0;
+      case OA_MU: HY_FILL(mu[p]); break;
+      case OA_U1: HY_FILL(m0[p] * inv[p]); break;
+      case OA_U2: HY_FILL(m1[p] * inv[p]); break;
+      case OA_U3: HY_FILL(m2[p] * inv[p]); break;
+      case OA_F11: HY_FILL(m0[p] * (m0[p] * inv[p])); break;
+      case OA_F12: HY_FILL(m0[p] * (m1[p] * inv[p])); break;
+      case OA_F13: HY_FILL(m0[p] * (m2[p] * inv[p])); break;
+      case OA_F22: HY_FILL(m1[p] * (m1[p] * inv[p])); break;
+      case OA_F23: HY_FILL(m1[p] * (m2[p] * inv[p])); break;
+      case OA_F33: HY_FILL(m2[p] * (m2[p] * inv[p])); break;
+#undef HY_FILL
+      default: break;
     }
   };
   for (int o = 0; o < a.nout; o += 2) {
     const bool two = o + 1 < a.nout;
-    const int ca = a.code[o], cb = two ? a.code[o + 1] : -1;
-    x_fwd_pair<T, NX, PTS>(buf, tw, [&](int p) { return mk<T>((T)val(ca, p), (T)val(cb, p)); }, a.dst[o],
-                           two ? a.dst[o + 1] : nullptr, line, a.Kx, lane);
+    T va[PTS], vb[PTS];
+    fill(a.code[o], va);
+    fill(two ? a.code[o + 1] : -1, vb);
+    X::fwd_pair(sm, [&](int q) { return mk<T>(va[q], vb[q]); }, a.dst[o], two ? a.dst[o + 1] : nullptr, line,
+                lok, a.Kx, g, lw);
   }
 }
 
-template <class T, int NX>
+template <class T, int NX, class X, bool MX>
 __global__ __launch_bounds__(64) void k_phaseB(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
-  constexpr int PTS = NX >= 64 ? NX / 64 : 1;
-  __shared__ cpx<T> buf[NX];
-  __shared__ cpx<T> tw[NX];
-  const PhysArgs& a = *pa;
-  const int lane = threadIdx.x;
-  const size_t line = blockIdx.x;
-  const int nd = a.nd;
-  const int iy = (int)(line % a.ny), iz = (int)(line / a.ny);
-  for (int j = lane; j < NX; j += 64) tw[j] = twg[j];
-  double dn[PTS], gmx[PTS], gmy[PTS], gmz[PTS];
-  x_inv_pair<T, NX>(buf, tw, &a.n, &a.mux, line, a.Kx, a.kx, lane);
-  x_get<T, NX, PTS>(buf, dn, gmx, lane);
-  if (nd >= 2) {
-    x_inv_pair<T, NX>(buf, tw, &a.muy, nd >= 3 ? &a.muz : nullptr, line, a.Kx, a.kx, lane);
-    x_get<T, NX, PTS>(buf, gmy, gmz, lane);
-  } else {
+  HY_XLINE_PREAMBLE
+  const int iy = lok ? (int)(line % a.ny) : 0, iz = lok ? (int)(line / a.ny) : 0;
+  T dn[PTS], gmx[PTS], gmy[PTS], gmz[PTS];
 #pragma unroll
-    for (int p = 0; p < PTS; ++p) { gmy[p] = 0.0; gmz[p] = 0.0; }
-  }
-  // external force (external_potential.f90:65-93), at the stage-start time
-  double f0[PTS], f1[PTS], f2[PTS];
-#pragma unroll
-  for (int p = 0; p < PTS; ++p) { f0[p] = 0.0; f1[p] = 0.0; f2[p] = 0.0; }
-  if (a.vext_kind == HY_VEXT_MODE || a.vext_kind == HY_VEXT_DRIVEN) {
-    const double t = a.vext_td ? a.ctl->ts : 0.0;
-    double amp = a.amp;
+  for (int p = 0; p < PTS; ++p) { gmy[p] = 0; gmz[p] = 0; }
+  X::inv_pair(sm, &a.n, &a.mux, line, lok, a.Kx, a.kx, g, lw, dn, gmx);
+  if (nd >= 2) X::inv_pair(sm, &a.muy, nd >= 3 ? &a.muz : nullptr, line, lok, a.Kx, a.kx, g, lw, gmy, gmz);
+  // external force (external_potential.f90:65-93) at the stage-start time,
+  // evaluated point by point
+  const bool wave_f = a.vext_kind == HY_VEXT_MODE || a.vext_kind == HY_VEXT_DRIVEN;
+  const bool gauss_f = a.vext_kind == HY_VEXT_GAUSSIAN && lok;
+  double amp = 0.0, t = 0.0, yz = 0.0;
+  if (wave_f) {
+    t = a.vext_td ? a.ctl->ts : 0.0;
+    amp = a.amp;
     if (a.vext_td && a.ramp > 0.0) amp = amp * fmin(1.0, t / a.ramp);
-    const double yz = a.kvec[1] * (iy * a.dy) + a.kvec[2] * (iz * a.dz);
-#pragma unroll
-    for (int p = 0; p < PTS; ++p) {
-      const int x = lane + 64 * p;
-      const double s = amp * sin(a.kvec[0] * (x * a.dx) + yz - a.omega * t);
-      f0[p] = s * a.kvec[0]; f1[p] = s * a.kvec[1]; f2[p] = s * a.kvec[2];
-    }
-  } else if (a.vext_kind == HY_VEXT_GAUSSIAN) {
-    const size_t nr = (size_t)NX * a.ny * a.nz;
-    const double* fe = a.fext + line * NX;
-#pragma unroll
-    for (int p = 0; p < PTS; ++p) {
-      const int x = lane + 64 * p;
-      if (x < NX) { f0[p] = fe[x]; f1[p] = fe[nr + x]; f2[p] = fe[2 * nr + x]; }
-    }
+    yz = a.kvec[1] * (iy * a.dy) + a.kvec[2] * (iz * a.dz);
   }
-  double h0[PTS], h1[PTS], h2[PTS];
+  const size_t nr = (size_t)NX * a.ny * a.nz;
+  const double* fe = a.fext + (lok ? line : 0) * NX;
+  T h0[PTS], h1[PTS], h2[PTS];
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
   for (int p = 0; p < PTS; ++p) {
-    const bool live = (lane + 64 * p) < NX;
+    const int x = X::xpos(g, p);
+    const bool live = lok && x >= 0;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    if (wave_f) {
+      const double sv = amp * sin(a.kvec[0] * (x * a.dx) + yz - a.omega * t);
+      f0 = sv * a.kvec[0]; f1 = sv * a.kvec[1]; f2 = sv * a.kvec[2];
+    } else if (gauss_f && x >= 0) {
+      f0 = fe[x]; f1 = fe[nr + x]; f2 = fe[2 * nr + x];
+    }
     const double n = a.n0 + dn[p];
-    h0[p] = n * (f0[p] - gmx[p]);
-    h1[p] = n * (f1[p] - gmy[p]);
-    h2[p] = n * (f2[p] - gmz[p]);
-    if (live) { s0 += n * f0[p]; s1 += n * f1[p]; s2 += n * f2[p]; }
+    h0[p] = (T)(n * (f0 - gmx[p]));
+    h1[p] = (T)(n * (f1 - gmy[p]));
+    h2[p] = (T)(n * (f2 - gmz[p]));
+    if (live) { s0 += n * f0; s1 += n * f1; s2 += n * f2; }
   }
   if (a.vext_kind != HY_VEXT_NONE) {
-    s0 = wave_sum(s0); s1 = wave_sum(s1); s2 = wave_sum(s2);
-    if (lane == 0) { a.red[line * 3] = s0; a.red[line * 3 + 1] = s1; a.red[line * 3 + 2] = s2; }
+    s0 = X::line_sum(s0); s1 = X::line_sum(s1); s2 = X::line_sum(s2);
+    if (g == 0 && lok) { a.red[line * 3] = s0; a.red[line * 3 + 1] = s1; a.red[line * 3 + 2] = s2; }
   }
-  auto hval = [&](int c, int p) -> double { return c == 0 ? h0[p] : (c == 1 ? h1[p] : (c == 2 ? h2[p] : 0.0)); };
-  for (int o = 0; o < nd; o += 2) {
-    const bool two = o + 1 < nd;
-    x_fwd_pair<T, NX, PTS>(buf, tw, [&](int p) { return mk<T>((T)hval(o, p), (T)hval(two ? o + 1 : -1, p)); },
-                           a.dst[o], two ? a.dst[o + 1] : nullptr, line, a.Kx, lane);
-  }
+  // H_1, H_2 share a transform (H_2 = 0 in 1-D and is not stored); H_3 goes alone
+  X::fwd_pair(sm, [&](int q) { return mk<T>(h0[q], h1[q]); }, a.dst[0], nd >= 2 ? a.dst[1] : nullptr, line, lok,
+              a.Kx, g, lw);
+  if (nd >= 3)
+    X::fwd_pair(sm, [&](int q) { return mk<T>(h2[q], T(0)); }, a.dst[2], nullptr, line, lok, a.Kx, g, lw);
+  if constexpr (MX) {
   if (!a.evolve) return;
   // Maxwell stress advection dPi_c = -sum_d u_d d_d Pi_c (model.f90:126-134),
   // two components per pass
-  double u0[PTS], u1[PTS], u2[PTS];
+  T u0[PTS], u1[PTS], u2[PTS];
   {
-    double m0[PTS], m1[PTS], m2[PTS];
-    x_inv_pair<T, NX>(buf, tw, &a.m[0], nd >= 2 ? &a.m[1] : nullptr, line, a.Kx, a.kx, lane);
-    x_get<T, NX, PTS>(buf, m0, m1, lane);
+    T m0[PTS], m1[PTS], m2[PTS];
 #pragma unroll
-    for (int p = 0; p < PTS; ++p) m2[p] = 0.0;
-    if (nd >= 3) {
-      x_inv_pair<T, NX>(buf, tw, &a.m[2], nullptr, line, a.Kx, a.kx, lane);
-      x_get<T, NX, PTS>(buf, m2, nullptr, lane);
-    }
+    for (int p = 0; p < PTS; ++p) { m1[p] = 0; m2[p] = 0; }
+    X::inv_pair(sm, &a.m[0], nd >= 2 ? &a.m[1] : nullptr, line, lok, a.Kx, a.kx, g, lw, m0, m1);
+    if (nd >= 3) X::inv_pair(sm, &a.m[2], nullptr, line, lok, a.Kx, a.kx, g, lw, m2, nullptr);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) {
       const double inv = 1.0 / (a.mass * (a.n0 + dn[p]));
@@ -427,64 +590,51 @@ __global__ __launch_bounds__(64) void k_phaseB(const PhysArgs* __restrict__ pa, 
     }
   }
   for (int c = 0; c < 6; c += 2) {
-    double xa[PTS], ya[PTS], za[PTS], xb[PTS], yb[PTS], zb[PTS];
+    T xa[PTS], ya[PTS], za[PTS], xb[PTS], yb[PTS], zb[PTS];
 #pragma unroll
-    for (int p = 0; p < PTS; ++p) { ya[p] = 0.0; za[p] = 0.0; yb[p] = 0.0; zb[p] = 0.0; }
+    for (int p = 0; p < PTS; ++p) { ya[p] = 0; za[p] = 0; yb[p] = 0; zb[p] = 0; }
     if (nd == 1) {
-      x_inv_pair<T, NX>(buf, tw, &a.px[c], &a.px[c + 1], line, a.Kx, a.kx, lane);
-      x_get<T, NX, PTS>(buf, xa, xb, lane);
+      X::inv_pair(sm, &a.px[c], &a.px[c + 1], line, lok, a.Kx, a.kx, g, lw, xa, xb);
     } else {
-      x_inv_pair<T, NX>(buf, tw, &a.px[c], &a.py[c], line, a.Kx, a.kx, lane);
-      x_get<T, NX, PTS>(buf, xa, ya, lane);
-      x_inv_pair<T, NX>(buf, tw, &a.px[c + 1], &a.py[c + 1], line, a.Kx, a.kx, lane);
-      x_get<T, NX, PTS>(buf, xb, yb, lane);
-      if (nd >= 3) {
-        x_inv_pair<T, NX>(buf, tw, &a.pz[c], &a.pz[c + 1], line, a.Kx, a.kx, lane);
-        x_get<T, NX, PTS>(buf, za, zb, lane);
-      }
+      X::inv_pair(sm, &a.px[c], &a.py[c], line, lok, a.Kx, a.kx, g, lw, xa, ya);
+      X::inv_pair(sm, &a.px[c + 1], &a.py[c + 1], line, lok, a.Kx, a.kx, g, lw, xb, yb);
+      if (nd >= 3) X::inv_pair(sm, &a.pz[c], &a.pz[c + 1], line, lok, a.Kx, a.kx, g, lw, za, zb);
     }
-    x_fwd_pair<T, NX, PTS>(
-        buf, tw,
+    X::fwd_pair(
+        sm,
         [&](int p) {
           const double va = -(u0[p] * xa[p]) - u1[p] * ya[p] - u2[p] * za[p];
           const double vb = -(u0[p] * xb[p]) - u1[p] * yb[p] - u2[p] * zb[p];
           return mk<T>((T)va, (T)vb);
         },
-        a.dst[nd + c], a.dst[nd + c + 1], line, a.Kx, lane);
+        a.dst[nd + c], a.dst[nd + c + 1], line, lok, a.Kx, g, lw);
+  }
   }
 }
 
 // max |u_d| over a line (model.f90:171-172)
-template <class T, int NX>
+template <class T, int NX, class X>
 __global__ __launch_bounds__(64) void k_umax(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
-  constexpr int PTS = NX >= 64 ? NX / 64 : 1;
-  __shared__ cpx<T> buf[NX];
-  __shared__ cpx<T> tw[NX];
-  const PhysArgs& a = *pa;
-  const int lane = threadIdx.x;
-  const size_t line = blockIdx.x;
-  for (int j = lane; j < NX; j += 64) tw[j] = twg[j];
-  double dn[PTS], m0[PTS], m1[PTS], m2[PTS];
+  HY_XLINE_PREAMBLE
+  T dn[PTS], m0[PTS], m1[PTS], m2[PTS];
 #pragma unroll
-  for (int p = 0; p < PTS; ++p) { m1[p] = 0.0; m2[p] = 0.0; }
-  x_inv_pair<T, NX>(buf, tw, &a.n, &a.m[0], line, a.Kx, a.kx, lane);
-  x_get<T, NX, PTS>(buf, dn, m0, lane);
-  if (a.nd >= 2) {
-    x_inv_pair<T, NX>(buf, tw, &a.m[1], a.nd >= 3 ? &a.m[2] : nullptr, line, a.Kx, a.kx, lane);
-    x_get<T, NX, PTS>(buf, m1, m2, lane);
-  }
+  for (int p = 0; p < PTS; ++p) { m1[p] = 0; m2[p] = 0; }
+  X::inv_pair(sm, &a.n, &a.m[0], line, lok, a.Kx, a.kx, g, lw, dn, m0);
+  if (nd >= 2) X::inv_pair(sm, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, line, lok, a.Kx, a.kx, g, lw, m1, m2);
   double um = 0.0;
 #pragma unroll
   for (int p = 0; p < PTS; ++p) {
-    if (lane + 64 * p >= NX) continue;
+    if (!lok || X::xpos(g, p) < 0) continue;
     const double inv = 1.0 / (a.mass * (a.n0 + dn[p]));
     um = fmax(um, fabs(m0[p] * inv));
-    if (a.nd >= 2) um = fmax(um, fabs(m1[p] * inv));
-    if (a.nd >= 3) um = fmax(um, fabs(m2[p] * inv));
+    if (nd >= 2) um = fmax(um, fabs(m1[p] * inv));
+    if (nd >= 3) um = fmax(um, fabs(m2[p] * inv));
   }
-  um = wave_max(um);
-  if (lane == 0) a.red[line] = um;
+  um = X::line_max(um);
+  if (g == 0 && lok) a.red[line] = um;
 }
+#undef HY_XLINE_PREAMBLE
+
 // compute_dt from the per-line maxima (model.f90:169-181)
 __global__ void k_dt(const double* __restrict__ red, int nlines, Ctl* c, double wmax, double numax,
                      double dxmin, double cfl) {
@@ -938,17 +1088,25 @@ static int fwd_chain(Model& m, const std::vector<std::pair<int, int>>& f /* (xsl
 enum XK { XK_A = 0, XK_B = 1, XK_UMAX = 2 };
 // device argument slots: one per distinct argument set
 enum XSlot { XA_STAGE = 0, XA_RELAX = 1, XB_STAGE = 2, XU_DT = 3, XA_N = 4 };
+template <class T, int NX, class X, bool MX>
+static void launch_xk_with(int which, const PhysArgs* a, int nlines, const cpx<T>* tw, hipStream_t s) {
+  const dim3 grid((unsigned)((nlines + X::LPW - 1) / X::LPW));
+  if (which == XK_A) hipLaunchKernelGGL((k_phaseA<T, NX, X>), grid, dim3(64), 0, s, a, tw);
+  else if (which == XK_B) hipLaunchKernelGGL((k_phaseB<T, NX, X, MX>), grid, dim3(64), 0, s, a, tw);
+  else hipLaunchKernelGGL((k_umax<T, NX, X>), grid, dim3(64), 0, s, a, tw);
+}
+// phase B without the Maxwell advection code where the closure has no
+// evolving stress (fewer registers)
 template <class T, int NX>
-static void launch_xk(int which, const PhysArgs* a, int nlines, const cpx<T>* tw, hipStream_t s) {
-  if (which == XK_A) hipLaunchKernelGGL((k_phaseA<T, NX>), dim3(nlines), dim3(64), 0, s, a, tw);
-  else if (which == XK_B) hipLaunchKernelGGL((k_phaseB<T, NX>), dim3(nlines), dim3(64), 0, s, a, tw);
-  else hipLaunchKernelGGL((k_umax<T, NX>), dim3(nlines), dim3(64), 0, s, a, tw);
+static void launch_xk(int which, bool maxwell, const PhysArgs* a, int nlines, const cpx<T>* tw, hipStream_t s) {
+  if (maxwell) launch_xk_with<T, NX, XWave<T, NX>, true>(which, a, nlines, tw, s);
+  else launch_xk_with<T, NX, XWave<T, NX>, false>(which, a, nlines, tw, s);
 }
 template <class T>
 static int dispatch_xk(int which, const Model& m, const PhysArgs* a, const cpx<T>* tw) {
   const int nlines = m.g.ny * m.g.nz;
   switch (m.g.nx) {
-#define HY_XK(LL) case LL: launch_xk<T, LL>(which, a, nlines, tw, m.st); break;
+#define HY_XK(LL) case LL: launch_xk<T, LL>(which, m.evolve != 0, a, nlines, tw, m.st); break;
     HY_XK(4) HY_XK(8) HY_XK(12) HY_XK(16) HY_XK(24) HY_XK(32) HY_XK(48) HY_XK(64) HY_XK(96)
     HY_XK(128) HY_XK(256)
 #undef HY_XK

@@ -36,15 +36,24 @@ CASES = {
     "ion3d_mode_nodealias": (dict(nx=16, ny=8, nz=8, lx=20.0, ly=10.0, lz=10.0, gamma=2.0, kappa=0.1,
                                   eta=0.5, vext="mode", vext_amplitude=1e-3, vext_mode=(1, 1, 0),
                                   dealias=False, dt=0.02), 3),
+    # x lines of 96 points (1.5 points per lane) and a 48-point y axis
+    "ion3d_nx96_visc": (dict(nx=96, ny=12, nz=8, lx=30.0, ly=9.0, lz=6.0, gamma=5.0, kappa=1.0,
+                             eta=0.3, xi=0.1, dt=0.01), 2),
+    "ion3d_maxwell_y48": (dict(nx=16, ny=48, nz=8, lx=12.0, ly=30.0, lz=6.0, gamma=20.0, kappa=1.0, eta=0.2,
+                               closure="maxwell", tau_model="constant", tau=0.5, dt=0.005), 2),
 }
 
+# FP32 transforms with the FP64 RK3 state: density error against the FP64
+# oracle after the case's steps (relative L2 of n, and of n - n0)
+MIXED_TOL_N, MIXED_TOL_DN = 1e-6, 1e-4
 
-def build(name):
+
+def build(name, precision="fp64"):
     from hydft_amd.model import Model
     kw, steps = CASES[name]
     p = P.params(**kw)
     st = P.structure(p.kind, p.gamma, p.kappa, p.rs)
-    gm = Model(p, structure=st, precision="fp64")
+    gm = Model(p, structure=st, precision=precision)
     om = P.oracle_model(p, st)
     evolve = om.stress.evolve
     n, mom, pi = P.smooth_fields(p, seed=zlib.crc32(name.encode()) % 1000, amp_n=0.02, amp_u=0.01, with_pi=evolve)
@@ -112,3 +121,19 @@ def test_mu_and_budgets_match_oracle(gpu, name):
         assert abs(mom[d] - omom[d]) <= 1e-12 * abs(omass) * 1e-2 + 1e-10 * abs(omom[d])
     assert abs(ekin - oekin) <= 1e-9 * abs(oekin)
     assert abs(fe - ofe) <= 1e-11 * abs(ofe)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_mixed_steps_match_oracle(gpu, name):
+    p, gm, om, steps = build(name, precision="mixed")
+    om.dt = p.dt
+    for _ in range(steps):
+        om.advance()
+    taken, _ = gm.advance(steps)
+    assert taken == steps
+    n, mom, pi = gm.get_state()
+    n0 = P.N0
+    assert P.rel_l2(n, om.n) <= MIXED_TOL_N, name
+    assert P.rel_l2(n - n0, om.n - n0) <= MIXED_TOL_DN, name
+    for d in range(p.ny > 1 and (3 if p.nz > 1 else 2) or 1):
+        assert P.rel_l2(mom[d], om.mom[d]) <= MIXED_TOL_DN, (name, d)
