@@ -11,18 +11,27 @@ dt is the CFL step (cfl = 0.4) at t = 0, frozen.  Synthetic, seeded inputs.
 
 One JSON line on stdout.  `roofline` is for the kernel class with the largest
 summed time in the timed region (HIP events on the model's stream; algorithmic
-bytes = compulsory reads + writes of each launch).  `cpu_baseline` times one
-RK3 stage of the CPU FP64 oracle (the reference dataflow, single core) on the
-same workload and scales it to steps/s.
+bytes = compulsory reads + writes of each launch).  Its `traffic` is the
+measured HBM bytes per launch of that kernel: two rocprofv3 counter runs
+(FETCH_SIZE, WRITE_SIZE) of a one-step child process, started before this
+process uses the GPU, each scaled by what the counter reads for a plain 1 GiB
+copy in the same child.  `cpu_baseline` times one RK3 stage of the CPU FP64
+oracle (the reference dataflow, single core) on the same workload and scales
+it to steps/s.
 """
 from __future__ import annotations
 
 import argparse
+import csv
 import ctypes as C
+import glob
 import json
 import math
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -95,6 +104,85 @@ def copy_probe(mib):
     return 0
 
 
+# kernel-class name -> device kernel of that class (pencil classes share one
+# kernel per direction across the y and z axes)
+CLASS_KERNEL = {"x_phaseA": "k_phaseA", "x_phaseB": "k_phaseB", "spec_update": "k_spec2", "spec_mu": "k_spec1",
+                "pencil_z_inverse": "k_pencil_inv", "pencil_y_inverse": "k_pencil_inv",
+                "pencil_y_forward": "k_pencil_fwd", "pencil_z_forward": "k_pencil_fwd", "x_umax": "k_umax"}
+COPY_MIB = 1024
+
+
+def pmc_child(args):
+    """Body of the counter runs: a plain copy of known size, then one RK3 step."""
+    import torch
+    from hydft_amd import abi
+    from hydft_amd.model import Model, build_species
+    copy_probe(COPY_MIB)
+    abi.call("hy_init", 0)
+    p = config(args.n, args.precision)
+    sp = build_species(p)
+    m = Model(p, species=sp, precision=args.precision)
+    n, mom = initial_state(p, m.n0)
+    m.set_state(n, mom)
+    m.set_dt(m.compute_dt())
+    m.advance(1)
+    torch.cuda.synchronize()
+    return 0
+
+
+def pmc_traffic(args):
+    """HBM bytes per launch of every kernel, from two rocprofv3 runs of the
+    one-step child (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch).  Each
+    counter is scaled by the ratio of known bytes to what it reads for the
+    child's plain 1 GiB copy.  Runs before this process touches the GPU."""
+    rocprof = shutil.which("rocprofv3")
+    if not rocprof:
+        return None, "rocprofv3 not found"
+    per = {}
+    factors = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="hydft_pmc_")
+        cmd = [rocprof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "pmc", "--",
+               sys.executable, os.path.abspath(__file__), "--pmc-child", "--n", str(args.n),
+               "--precision", args.precision]
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
+        except subprocess.TimeoutExpired:
+            shutil.rmtree(d, ignore_errors=True)
+            return None, f"{counter} run timed out"
+        files = glob.glob(os.path.join(d, "**", "pmc_counter_collection.csv"), recursive=True)
+        if r.returncode != 0 or not files:
+            shutil.rmtree(d, ignore_errors=True)
+            return None, f"{counter} run failed (rc {r.returncode})"
+        vals = {}
+        for row in csv.DictReader(open(files[0])):
+            vals.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]) * 1024.0)
+        shutil.rmtree(d, ignore_errors=True)
+        copies = [v for k, vs in vals.items() if "copyBuffer" in k for v in vs]
+        if not copies:
+            return None, "no copy kernel in the counter run"
+        known = COPY_MIB * 1048576.0
+        factors[counter] = known / max(copies)
+        for k, vs in vals.items():
+            per.setdefault(k, {})[counter] = sum(vs) / len(vs) * factors[counter]
+    return {"per_kernel": per, "factors": factors}, None
+
+
+def kernel_traffic(tr, cls, precision):
+    """(fetch + write) bytes per launch of the device kernel of class `cls` in
+    the run's transform type (pencil classes: mean over the y and z passes)."""
+    if not tr or cls not in CLASS_KERNEL:
+        return None
+    want = "<float" if precision == "mixed" else "<double"
+    names = [k for k in tr["per_kernel"] if CLASS_KERNEL[cls] + "<" in k]
+    names = [k for k in names if want in k] or names
+    for name in names:
+        c = tr["per_kernel"][name]
+        if "FETCH_SIZE" in c and "WRITE_SIZE" in c:
+            return c["FETCH_SIZE"] + c["WRITE_SIZE"]
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,15 +194,20 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter runs")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        return pmc_child(args)
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; runs on more than one GPU are not part of it "
               "(independent runs would shard with no exchange).", file=sys.stderr)
         return 1
 
-    import torch
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    traffic, traffic_note = (None, "skipped") if args.no_traffic else pmc_traffic(args)
+    import torch
     from hydft_amd import abi
     from hydft_amd.model import Model, build_species
 
@@ -151,6 +244,12 @@ def main():
                           "GBps": r["bytes"] / (r["total_ms"] * 1e-3) / 1e9} for r in prof],
              "step_hbm_GBps": tot_bytes / (tot_ms * 1e-3) / 1e9,
              "bytes_per_step": tot_bytes / args.steps}
+    if traffic:
+        extra["traffic_per_launch"] = {r["kernel"]: kernel_traffic(traffic, r["kernel"], args.precision)
+                                       for r in prof}
+        extra["counter_scale"] = traffic["factors"]
+    else:
+        extra["traffic_note"] = traffic_note
     # FP64 comparison on the same input: speed and the density error of the mixed run
     if not args.no_fp64 and args.precision == "mixed":
         nm, _, _ = m.get_state()
@@ -191,7 +290,8 @@ def main():
                    "rs": p.rs, "theta": p.theta, "terms": p.terms, "closure": p.closure,
                    "transport": p.transport, "precision": args.precision, "dt": dt},
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": achieved, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": kernel_traffic(traffic, dom["kernel"], args.precision),
                      "bytes_per_launch": bpl, "avg_launch_us": avg_ms * 1e3},
         "cpu_baseline": cpu,
         "extra": extra,

@@ -1116,7 +1116,9 @@ static int dispatch_xk(int which, const Model& m, const PhysArgs* a, const cpx<T
   HY_TRACE(which == XK_A ? "x_phaseA" : (which == XK_B ? "x_phaseB" : "x_umax"), m.st);
   return HY_OK;
 }
-static double xk_bytes(const Model& m, const PhysArgs& a) {
+// Compulsory bytes of one x-kernel launch: every distinct input x-line the
+// kernel reads once, every output written once.
+static double xk_bytes(const Model& m, int which, const PhysArgs& a) {
   std::vector<const void*> seen;
   double b = 0.0;
   const double line = (double)m.g.Kx * m.g.ny * m.g.nz;
@@ -1126,9 +1128,16 @@ static double xk_bytes(const Model& m, const PhysArgs& a) {
     seen.push_back(x.p);
     b += line * (x.is64 ? 16.0 : (double)m.esz);
   };
-  in(a.n); for (int d = 0; d < 3; ++d) in(a.m[d]);
-  in(a.gx); in(a.gy); in(a.gz); in(a.lap); in(a.mux); in(a.muy); in(a.muz);
-  for (int c = 0; c < 6; ++c) { in(a.px[c]); in(a.py[c]); in(a.pz[c]); }
+  const int nd = m.g.ndim;
+  in(a.n);
+  if (which == XK_A || which == XK_UMAX || a.evolve)
+    for (int d = 0; d < nd; ++d) in(a.m[d]);
+  if (which == XK_A && a.grad) { in(a.gx); in(a.gy); in(a.gz); in(a.lap); }
+  if (which == XK_B) {
+    in(a.mux); in(a.muy); in(a.muz);
+    if (a.evolve)
+      for (int c = 0; c < 6; ++c) { in(a.px[c]); in(a.py[c]); in(a.pz[c]); }
+  }
   b += line * a.nout * (double)m.esz;
   return b;
 }
@@ -1149,7 +1158,7 @@ static int run_xk(Model& m, int which, int slot, const PhysArgs& a) {
   const TwiddleSet& t = m.g.tw(m.g.nx);
   const PhysArgs* da = nullptr;
   HY_TRY(xargs_dev(m, slot, a, &da));
-  prof_begin(m, which == XK_A ? PC_XA : (which == XK_B ? PC_XB : PC_UMAX), xk_bytes(m, a));
+  prof_begin(m, which == XK_A ? PC_XA : (which == XK_B ? PC_XB : PC_UMAX), xk_bytes(m, which, a));
   int rc = mixed(m) ? dispatch_xk<float>(which, m, da, t.d32) : dispatch_xk<double>(which, m, da, t.d64);
   prof_end(m);
   return rc;
