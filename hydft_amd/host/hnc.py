@@ -276,3 +276,88 @@ def structure_from_file(path, kind, n0) -> Structure:
     if k[0] <= 0.0:
         raise HncError("structure file: k must be positive")
     return Structure(k=k, c=c, n0=n0, source="file")
+
+
+# ------------------------------------------------------------ c(k) supply
+def _structure_key(pot: PairPotential, n0, nr, rmax, tol, mix, maxiter, with_derivative) -> tuple:
+    return (pot.name, float(pot.gamma), float(pot.kappa), float(pot.ksplit), float(pot.lam), bool(pot.pauli),
+            float(n0), int(nr), float(rmax), float(tol), float(mix), int(maxiter), bool(with_derivative))
+
+
+class StructureCache:
+    """c(k) tables keyed by the potential, the state point and the solver
+    settings, in memory and optionally on disk (one .npz per key, written and
+    read by this class only).  Batched and served runs reuse a solve whenever
+    the key repeats; `solve_many` fills the cache for a batch of distinct
+    keys in parallel processes."""
+
+    def __init__(self, directory=None, maxsize=1024):
+        self.directory = directory
+        self.maxsize = maxsize
+        self._mem = {}
+        self.hits = self.misses = 0
+        if directory:
+            import os
+            os.makedirs(directory, exist_ok=True)
+
+    def _path(self, key):
+        import hashlib
+        import os
+        h = hashlib.sha1(repr(key).encode()).hexdigest()[:24]
+        return os.path.join(self.directory, f"ck_{h}.npz")
+
+    def _load(self, key):
+        import os
+        if not self.directory:
+            return None
+        path = self._path(key)
+        if not os.path.exists(path):
+            return None
+        z = np.load(path, allow_pickle=False)
+        if tuple(z["key"].tolist()) != tuple(str(v) for v in key):
+            return None
+        return Structure(k=z["k"], c=z["c"], n0=float(z["n0"]), u_ex=float(z["u_ex"]),
+                         du_dgamma=float(z["du_dgamma"]), has_energy=True)
+
+    def _store(self, key, st):
+        self._mem[key] = st
+        if len(self._mem) > self.maxsize:
+            self._mem.pop(next(iter(self._mem)))
+        if self.directory:
+            np.savez(self._path(key), key=np.array([str(v) for v in key]), k=st.k, c=st.c, n0=st.n0,
+                     u_ex=st.u_ex, du_dgamma=st.du_dgamma)
+
+    def get(self, pot: PairPotential, n0, nr=8192, rmax=80.0, tol=1.0e-9, mix=0.2, maxiter=20000,
+            with_derivative=False) -> Structure:
+        key = _structure_key(pot, n0, nr, rmax, tol, mix, maxiter, with_derivative)
+        st = self._mem.get(key)
+        if st is None:
+            st = self._load(key)
+            if st is not None:
+                self._mem[key] = st
+        if st is not None:
+            self.hits += 1
+            return st
+        self.misses += 1
+        st = structure_from_hnc(pot, n0, nr, rmax, tol, mix, maxiter, with_derivative)
+        self._store(key, st)
+        return st
+
+    def solve_many(self, requests, processes=None):
+        """requests: (pot, n0, nr, rmax, tol, mix, maxiter, with_derivative)
+        tuples.  Solves the keys not yet cached in a process pool and returns
+        the Structures in request order."""
+        import concurrent.futures as cf
+        keys = [_structure_key(*r) for r in requests]
+        todo = {}
+        for key, r in zip(keys, requests):
+            if key not in self._mem and self._load(key) is None and key not in todo:
+                todo[key] = r
+        if todo:
+            workers = processes or min(len(todo), 8)
+            with cf.ProcessPoolExecutor(max_workers=workers) as ex:
+                futs = {key: ex.submit(structure_from_hnc, *r) for key, r in todo.items()}
+                for key, f in futs.items():
+                    self._store(key, f.result())
+                    self.misses += 1
+        return [self.get(*r) for r in requests]

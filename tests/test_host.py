@@ -140,3 +140,22 @@ def test_analysis_recovers_known_signals():
     z = a0 * np.exp(-1j * w * t)
     a, sc = fit_driven_response(t, z, w, 30.0)
     assert abs(a - a0) < 1e-15 and sc < 1e-9
+
+
+def test_structure_cache(tmp_path):
+    n0 = 3.0 / (4.0 * np.pi)
+    cache = hnc.StructureCache(directory=str(tmp_path))
+    pot = hnc.PairPotential("yukawa", 5.0, 1.0)
+    a = cache.get(pot, n0, nr=1024, rmax=30.0, tol=1e-8)
+    b = cache.get(hnc.PairPotential("yukawa", 5.0, 1.0), n0, nr=1024, rmax=30.0, tol=1e-8)
+    assert b is a and cache.hits == 1 and cache.misses == 1
+    # a new process-level cache finds the disk entry and returns the same table
+    c = hnc.StructureCache(directory=str(tmp_path)).get(pot, n0, nr=1024, rmax=30.0, tol=1e-8)
+    np.testing.assert_array_equal(c.c, a.c)
+    np.testing.assert_array_equal(c.k, a.k)
+    # solve_many: distinct keys solved once each, order kept, equal to direct solves
+    reqs = [(hnc.PairPotential("yukawa", g, 1.0), n0, 1024, 30.0, 1e-8, 0.2, 20000, False) for g in (2.0, 5.0, 2.0)]
+    out = cache.solve_many(reqs, processes=2)
+    direct = hnc.structure_from_hnc(hnc.PairPotential("yukawa", 2.0, 1.0), n0, 1024, 30.0, 1e-8)
+    np.testing.assert_allclose(out[0].c, direct.c, rtol=0, atol=1e-12)
+    assert out[2] is out[0] and out[1] is cache.get(pot, n0, nr=1024, rmax=30.0, tol=1e-8)
