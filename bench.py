@@ -183,6 +183,45 @@ def kernel_traffic(tr, cls, precision):
     return None
 
 
+def serve_sample(nreq, steps, grid=32):
+    """Batched / served chi(k, w) throughput against one run at a time, on a
+    seeded sample of driven-mode requests (Yukawa ions, Gamma in [1, 100],
+    kappa in [0.5, 3], Newtonian / Maxwell alternating, CFL dt), FP32-mixed.
+    HNC c(k) are solved (in parallel host processes) before the timed parts."""
+    from hydft_amd import serve
+    from hydft_amd.host import hnc
+    from hydft_amd.model import hnc_request
+    reqs = serve.chi_requests(nreq, seed=11, grid=(grid,), steps=steps)
+    cache = hnc.StructureCache()
+    t0 = time.perf_counter()
+    cache.solve_many([hnc_request(r.p) for r in reqs])
+    t_hnc = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    one = serve.one_at_a_time(reqs, precision="mixed", cache=cache)
+    w_one = time.perf_counter() - t0
+    srv = serve.Server(precision="mixed", cache=cache, max_slots=64)
+    t0 = time.perf_counter()
+    batch = srv.serve(reqs, realtime=False)
+    w_batch = time.perf_counter() - t0
+    # served stream: Poisson arrivals at 50 requests/s
+    reqs_rt = serve.chi_requests(nreq, seed=11, grid=(grid,), steps=steps, arrival_rate=50.0)
+    srv_rt = serve.Server(precision="mixed", cache=cache, max_slots=64)
+    t0 = time.perf_counter()
+    served = srv_rt.serve(reqs_rt, realtime=True)
+    w_served = time.perf_counter() - t0
+    same = all(np.array_equal(a.probe, b.probe) for a, b in
+               zip(sorted(one, key=lambda r: r.rid), sorted(batch, key=lambda r: r.rid)))
+    s_one, s_b, s_s = serve.summary(one, w_one), serve.summary(batch, w_batch), serve.summary(served, w_served)
+    return {"requests": nreq, "grid": grid, "steps_per_run": steps, "hnc_presolve_s": t_hnc,
+            "one_at_a_time_runs_per_s": s_one["runs_per_s"], "batched_runs_per_s": s_b["runs_per_s"],
+            "batched_over_one_at_a_time": s_b["runs_per_s"] / s_one["runs_per_s"],
+            "batched_peak_active": srv.peak_active,
+            "served_arrival_rate_per_s": 50.0, "served_runs_per_s": s_s["runs_per_s"],
+            "served_latency_p50_s": s_s["latency_p50_s"], "served_latency_p99_s": s_s["latency_p99_s"],
+            "one_at_a_time_latency_p50_s": s_one["latency_p50_s"],
+            "batched_probe_series_equal_one_at_a_time": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +234,9 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter runs")
+    ap.add_argument("--serve", type=int, default=32, metavar="NREQ",
+                    help="batched/served chi(k, w) sample size (0: skip)")
+    ap.add_argument("--serve-steps", type=int, default=400)
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -269,6 +311,8 @@ def main():
         extra["mixed_dn_rel_l2_vs_fp64"] = float(np.linalg.norm(nm - n64) / np.linalg.norm(n64 - m.n0))
         extra["steps_compared"] = args.warmup + args.steps
         m64.close()
+    if args.serve > 0:
+        extra["serve"] = serve_sample(args.serve, args.serve_steps)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(p, st, n, mom)

@@ -868,6 +868,22 @@ __global__ void k_probe(const Ctl* c, const c64* S, size_t nt, long long idx, do
   out[5] = c->dt;
 }
 
+// the same row into a ring of `ring` rows at slot (step - 1) % ring, where
+// step is the device step count after the step (captured steps: every replay
+// appends its rows)
+__global__ void k_probe_ring(const Ctl* c, const c64* S, size_t nt, long long idx, double n0, double* out,
+                             int ring) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int slot = (int)(((long long)c->step - 1) % ring + ring) % ring;
+  c64 a = idx >= 0 ? S[idx] : mk<double>(0, 0);
+  c64 b = idx >= 0 ? S[nt + idx] : mk<double>(0, 0);
+  double* o = out + 6 * (size_t)slot;
+  o[0] = c->t;
+  o[1] = a.x / n0; o[2] = a.y / n0;
+  o[3] = b.x; o[4] = b.y;
+  o[5] = c->dt;
+}
+
 // full half spectrum <-> kept box
 __global__ void k_full_to_trunc(const c64* __restrict__ full, c64* __restrict__ tr, SpecArgs a, int nkx) {
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < a.nt; e += (size_t)gridDim.x * blockDim.x) {
@@ -921,8 +937,17 @@ struct Model {
   double* red = nullptr;
   Ctl* ctl = nullptr;
   PhysArgs* d_xargs = nullptr;  // XA_N device copies of the x-kernel arguments
-  PhysArgs h_xargs[4];          // what each device copy holds
+  PhysArgs* p_xargs = nullptr;  // pinned host staging of the same (source of the uploads)
+  // transfer buffers of set_state / get_state (a real field and its full
+  // half spectrum), allocated on first use and kept: freeing device memory
+  // waits for all work on the device, which would stall other models' streams
+  double* d_real = nullptr;
+  c64* d_full = nullptr;
   bool xargs_ok[4] = {false, false, false, false};
+  // captured RK3 steps (hy_model_graph_*)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int g_nsteps = 0, g_probe = 0, g_ring = 0;
   double* probe = nullptr;
   int probe_cap = 0;
   int nxslots = 0, nzslots = 0, nfslots = 0;
@@ -1142,13 +1167,22 @@ static double xk_bytes(const Model& m, int which, const PhysArgs& a) {
   return b;
 }
 // Device copy of an argument set.  The sets depend only on the model's
-// configuration, so each slot is uploaded on its first use; a changed set is
-// uploaded after the stream has drained (no launch still reads the old one).
+// configuration, so each slot is uploaded on its first use, from pinned host
+// staging and in stream order.  Inside a graph capture the upload becomes a
+// copy node of the graph (its source stays valid and unchanged); a set that
+// changes while a capture is active is an error.
 static int xargs_dev(Model& m, int slot, const PhysArgs& a, const PhysArgs** out) {
-  if (!m.xargs_ok[slot] || std::memcmp(&m.h_xargs[slot], &a, sizeof(PhysArgs)) != 0) {
-    HY_HIP(hipStreamSynchronize(m.st));
-    HY_HIP(hipMemcpy(m.d_xargs + slot, &a, sizeof(PhysArgs), hipMemcpyHostToDevice));
-    std::memcpy(&m.h_xargs[slot], &a, sizeof(PhysArgs));
+  const bool same = m.xargs_ok[slot] && std::memcmp(&m.p_xargs[slot], &a, sizeof(PhysArgs)) == 0;
+  if (!same) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HY_HIP(hipStreamIsCapturing(m.st, &cs));
+    if (cs == hipStreamCaptureStatusActive) {
+      if (m.xargs_ok[slot]) return set_error(HY_ERR_INVALID, "x-kernel arguments changed during graph capture");
+    } else {
+      HY_HIP(hipStreamSynchronize(m.st));  // no queued upload still reads the staging slot
+    }
+    std::memcpy(&m.p_xargs[slot], &a, sizeof(PhysArgs));
+    HY_HIP(hipMemcpyAsync(m.d_xargs + slot, &m.p_xargs[slot], sizeof(PhysArgs), hipMemcpyHostToDevice, m.st));
     m.xargs_ok[slot] = true;
   }
   *out = m.d_xargs + slot;
@@ -1468,8 +1502,15 @@ static double fun_kernel(const Model& m, double k) {
 
 static void model_free(Model& m) {
   for (void* p : {(void*)m.S, (void*)m.R, (void*)m.uold, (void*)m.kcomb, (void*)m.fext, (void*)m.fdtab,
-                  m.xb, m.zb, m.fb, (void*)m.red, (void*)m.ctl, (void*)m.probe, (void*)m.d_xargs})
+                  m.xb, m.zb, m.fb, (void*)m.red, (void*)m.ctl, (void*)m.probe, (void*)m.d_xargs,
+                  (void*)m.d_real, (void*)m.d_full})
     if (p) (void)hipFree(p);
+  if (m.gexec) (void)hipGraphExecDestroy(m.gexec);
+  if (m.graph) (void)hipGraphDestroy(m.graph);
+  m.gexec = nullptr;
+  m.graph = nullptr;
+  if (m.p_xargs) (void)hipHostFree(m.p_xargs);
+  m.p_xargs = nullptr;
   for (auto& e : m.evs) { (void)hipEventDestroy(e.a); (void)hipEventDestroy(e.b); }
   for (auto e : m.pool) (void)hipEventDestroy(e);
   m.evs.clear();
@@ -1826,6 +1867,12 @@ int model_build(Model& m, const hy_model_desc* d) {
     m.dadn = 2.0 / (m.cn * fd::im12(tab, m.alpha0));
   }
   HY_TRY(dmalloc((void**)&m.d_xargs, sizeof(PhysArgs) * XA_N));
+  if (hipHostMalloc((void**)&m.p_xargs, sizeof(PhysArgs) * XA_N, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    m.p_xargs = nullptr;
+    return set_error(HY_ERR_OOM, "model: pinned host allocation failed");
+  }
+  std::memset(m.p_xargs, 0, sizeof(PhysArgs) * XA_N);
   // combined convolution kernel on the kept box, and compute_dt constants
   std::vector<double> kc(m.nt, 0.0);
   double wmax = 0.0;
@@ -1908,6 +1955,14 @@ int hy_model_destroy(hy_model h) {
   return HY_OK;
 }
 
+static int transfer_bufs(Model& m, double** dr, c64** df) {
+  if (!m.d_real) HY_TRY(dmalloc((void**)&m.d_real, sizeof(double) * m.g.nreal()));
+  if (!m.d_full) HY_TRY(dmalloc((void**)&m.d_full, sizeof(c64) * m.g.nfull()));
+  *dr = m.d_real;
+  *df = m.d_full;
+  return HY_OK;
+}
+
 // Real fields -> kept-box spectra, in FP64 (the state is FP64 in every mode).
 int hy_model_set_state(hy_model h, const double* h_n, const double* h_mom3, const double* h_pi6) {
   if (!h || !h_n) return set_error(HY_ERR_INVALID, "null argument");
@@ -1916,9 +1971,8 @@ int hy_model_set_state(hy_model h, const double* h_n, const double* h_mom3, cons
   const size_t nr = g.nreal();
   double* dr;
   c64* df;
-  HY_TRY(dmalloc((void**)&dr, sizeof(double) * nr));
-  int rc = dmalloc((void**)&df, sizeof(c64) * g.nfull());
-  if (rc) { (void)hipFree(dr); return rc; }
+  int rc = HY_OK;
+  HY_TRY(transfer_bufs(m, &dr, &df));
   SpecArgs sa = spec_args(m);
   auto put = [&](const double* src, int f) -> int {
     HY_HIP(hipMemcpyAsync(dr, src, sizeof(double) * nr, hipMemcpyHostToDevice, m.st));
@@ -1938,8 +1992,6 @@ int hy_model_set_state(hy_model h, const double* h_n, const double* h_mom3, cons
       else rc = hipMemsetAsync(sfield(m, 4 + c), 0, sizeof(c64) * m.nt, m.st) == hipSuccess ? HY_OK : HY_ERR_HIP;
     }
   (void)hipStreamSynchronize(m.st);
-  (void)hipFree(dr);
-  (void)hipFree(df);
   m.relax_pending = 0;
   return rc;
 }
@@ -1949,17 +2001,14 @@ static int trunc_to_real(Model& m, const c64* tr, double* h_out) {
   const size_t nr = g.nreal();
   double* dr;
   c64* df;
-  HY_TRY(dmalloc((void**)&dr, sizeof(double) * nr));
-  int rc = dmalloc((void**)&df, sizeof(c64) * g.nfull());
-  if (rc) { (void)hipFree(dr); return rc; }
+  int rc = HY_OK;
+  HY_TRY(transfer_bufs(m, &dr, &df));
   SpecArgs sa = spec_args(m);
   hipLaunchKernelGGL(k_trunc_to_full, dim3(nblocks(g.nfull())), dim3(256), 0, m.st, tr, df, sa, g.nkx);
   rc = fft_c2r_full_api(g, df, dr, m.st);
   if (rc == HY_OK && hipMemcpyAsync(h_out, dr, sizeof(double) * nr, hipMemcpyDeviceToHost, m.st) != hipSuccess)
     rc = set_error(HY_ERR_HIP, "copy");
   if (hipStreamSynchronize(m.st) != hipSuccess) rc = set_error(HY_ERR_HIP, "sync");
-  (void)hipFree(dr);
-  (void)hipFree(df);
   return rc;
 }
 
@@ -2079,6 +2128,126 @@ int hy_model_advance(hy_model h, int nsteps, int dt_every, double tend, const in
   if (h_taken) *h_taken = taken;
   if (c.nonpos) return set_error(HY_ERR_NONPOSITIVE, "density became non-positive; reduce the amplitude or the time step");
   if (c.nodt) return set_error(HY_ERR_NO_DT, "cannot determine a time step; set dt in &time");
+  return HY_OK;
+}
+
+// ---- captured steps
+int hy_model_graph_build(hy_model h, int nsteps, int dt_every, const int* probe3, int ring) {
+  if (!h || nsteps < 1 || dt_every < 0 || (probe3 && ring < nsteps))
+    return set_error(HY_ERR_INVALID, "hy_model_graph_build: bad argument");
+  Model& m = h->m;
+  const Grid& g = m.g;
+  if (dt_every > 0 && nsteps % dt_every != 0)
+    return set_error(HY_ERR_INVALID, "hy_model_graph_build: nsteps must be a multiple of dt_every");
+  long long pidx = -1;
+  if (probe3) {
+    int mx = probe3[0], my = probe3[1], mz = probe3[2];
+    if (mx < 0 || mx >= g.nkx) return set_error(HY_ERR_INVALID, "probe: mx out of range");
+    int wy = ((my % g.ny) + g.ny) % g.ny, wz = ((mz % g.nz) + g.nz) % g.nz;
+    int sy = wy <= g.ny / 2 ? wy : wy - g.ny, sz = wz <= g.nz / 2 ? wz : wz - g.nz;
+    if (mx < g.Kx && std::abs(sy) <= g.My && std::abs(sz) <= g.Mz) {
+      int ty = sy >= 0 ? sy : sy + g.Ky, tz = sz >= 0 ? sz : sz + g.Kz;
+      pidx = ((long long)tz * g.Ky + ty) * g.Kx + mx;
+    }
+    if (m.probe_cap < ring) {
+      HY_HIP(hipStreamSynchronize(m.st));
+      if (m.probe) (void)hipFree(m.probe);
+      m.probe = nullptr;
+      m.probe_cap = 0;
+      HY_TRY(dmalloc((void**)&m.probe, sizeof(double) * 6 * ring));
+      m.probe_cap = ring;
+    }
+  }
+  // the captured sequence starts and ends with no pending relaxation and at
+  // the start of the dt cadence, so replays chain like hy_model_advance calls
+  HY_TRY(finish_relax(m));
+  if (dt_every > 0) m.irecompute = 0;
+  HY_HIP(hipStreamSynchronize(m.st));
+  if (m.gexec) { (void)hipGraphExecDestroy(m.gexec); m.gexec = nullptr; }
+  if (m.graph) { (void)hipGraphDestroy(m.graph); m.graph = nullptr; }
+  const bool prof = m.prof;
+  m.prof = false;
+  HY_HIP(hipStreamBeginCapture(m.st, hipStreamCaptureModeThreadLocal));
+  int rc = HY_OK;
+  for (int i = 0; i < nsteps && rc == HY_OK; ++i) {
+    if (dt_every > 0) {
+      if (m.irecompute == 0) {
+        rc = finish_relax(m);
+        if (rc == HY_OK) rc = compute_dt_dev(m);
+      }
+      m.irecompute = (m.irecompute + 1) % dt_every;
+    }
+    if (rc == HY_OK) rc = step_once(m, 0.0);
+    if (rc == HY_OK && probe3) {
+      hipLaunchKernelGGL(k_probe_ring, dim3(1), dim3(1), 0, m.st, m.ctl, m.S, m.nt, pidx, m.n0, m.probe, ring);
+      if (hipGetLastError() != hipSuccess) rc = set_error(HY_ERR_HIP, "probe launch failed during capture");
+    }
+  }
+  if (rc == HY_OK) rc = finish_relax(m);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(m.st, &graph);
+  m.prof = prof;
+  if (rc != HY_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (ec != hipSuccess) return set_error(HY_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  m.graph = graph;
+  HY_HIP(hipGraphInstantiate(&m.gexec, m.graph, nullptr, nullptr, 0));
+  m.g_nsteps = nsteps;
+  m.g_probe = probe3 ? 1 : 0;
+  m.g_ring = probe3 ? ring : 0;
+  return HY_OK;
+}
+
+int hy_model_graph_launch(hy_model h, int nlaunch) {
+  if (!h || nlaunch < 0) return set_error(HY_ERR_INVALID, "hy_model_graph_launch: bad argument");
+  Model& m = h->m;
+  if (!m.gexec) return set_error(HY_ERR_INVALID, "hy_model_graph_launch: no graph (hy_model_graph_build)");
+  for (int i = 0; i < nlaunch; ++i) HY_HIP(hipGraphLaunch(m.gexec, m.st));
+  return HY_OK;
+}
+
+int hy_model_graph_probe(hy_model h, double* h_probe) {
+  if (!h || !h_probe) return set_error(HY_ERR_INVALID, "hy_model_graph_probe: null argument");
+  Model& m = h->m;
+  if (!m.gexec || !m.g_probe) return set_error(HY_ERR_INVALID, "hy_model_graph_probe: graph has no probe");
+  HY_HIP(hipStreamSynchronize(m.st));
+  HY_HIP(hipMemcpy(h_probe, m.probe, sizeof(double) * 6 * m.g_ring, hipMemcpyDeviceToHost));
+  return HY_OK;
+}
+
+int hy_model_idle(hy_model h, int* idle) {
+  if (!h || !idle) return set_error(HY_ERR_INVALID, "hy_model_idle: null argument");
+  const hipError_t e = hipStreamQuery(h->m.st);
+  if (e == hipSuccess) { *idle = 1; return HY_OK; }
+  if (e == hipErrorNotReady) { (void)hipGetLastError(); *idle = 0; return HY_OK; }
+  return set_error(HY_ERR_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
+int hy_model_check(hy_model h) {
+  if (!h) return set_error(HY_ERR_INVALID, "null argument");
+  Ctl c;
+  HY_TRY(read_ctl(h->m, c));
+  if (c.nonpos) return set_error(HY_ERR_NONPOSITIVE, "density became non-positive; reduce the amplitude or the time step");
+  if (c.nodt) return set_error(HY_ERR_NO_DT, "cannot determine a time step; set dt in &time");
+  return HY_OK;
+}
+
+int hy_model_device_bytes(hy_model h, unsigned long long* bytes) {
+  if (!h || !bytes) return set_error(HY_ERR_INVALID, "null argument");
+  const Model& m = h->m;
+  const Grid& g = m.g;
+  double b = sizeof(c64) * (double)m.nt * (10 + 10 + 3 + 10);             // S, R, u_old, rhs slots
+  b += (double)m.esz * ((double)m.xsz * m.nxslots + (double)m.zsz * m.nzslots + (double)m.nt * m.nfslots);
+  b += sizeof(double) * ((double)m.nt + std::max<double>((double)g.ny * g.nz * 4, 4096));  // kcomb, reductions
+  if (m.fext) b += sizeof(double) * 3.0 * g.nreal();
+  if (m.fdtab) b += sizeof(double) * (double)fd::NTAB * fd::NC;
+  b += sizeof(double) * 6.0 * m.probe_cap + sizeof(PhysArgs) * XA_N + sizeof(Ctl);
+  b += (double)g.scratch_bytes;
+  if (m.d_real) b += sizeof(double) * (double)g.nreal();
+  if (m.d_full) b += sizeof(c64) * (double)g.nfull();
+  *bytes = (unsigned long long)b;
   return HY_OK;
 }
 

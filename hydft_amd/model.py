@@ -82,19 +82,27 @@ class Species:
     evolve: bool
 
 
-def build_species(p: Params, structure=None) -> Species:
-    """species_build (species.f90:49-153)."""
+def hnc_request(p: Params):
+    """The HNC solve species_build asks for: (potential, n0, nr, rmax, tol, mix,
+    maxiter, with_derivative), the arguments of hnc.structure_from_hnc."""
     un = Units(p.kind, p.gamma, p.kappa, p.rs, p.mass)
+    if un.quantum:
+        pot = _hnc.PairPotential("hansen_mcdonald", un.gamma, 0.0, lam=un.lambda_qsp)
+    else:
+        pot = _hnc.PairPotential("yukawa", un.gamma, un.kappa)
     need_deriv = p.closure == "maxwell" and p.tau_model == "ichimaru"
+    return (pot, un.n0, p.hnc_nr, p.hnc_rmax, p.hnc_tol, p.hnc_mix, p.hnc_maxiter, need_deriv)
+
+
+def build_species(p: Params, structure=None, cache=None) -> Species:
+    """species_build (species.f90:49-153).  `cache`: an hnc.StructureCache
+    that supplies (and keeps) the HNC c(k)."""
+    un = Units(p.kind, p.gamma, p.kappa, p.rs, p.mass)
     st = structure
     if st is None:
         if p.structure == "hnc":
-            if un.quantum:
-                pot = _hnc.PairPotential("hansen_mcdonald", un.gamma, 0.0, lam=un.lambda_qsp)
-            else:
-                pot = _hnc.PairPotential("yukawa", un.gamma, un.kappa)
-            st = _hnc.structure_from_hnc(pot, un.n0, p.hnc_nr, p.hnc_rmax, p.hnc_tol, p.hnc_mix,
-                                         p.hnc_maxiter, with_derivative=need_deriv)
+            req = hnc_request(p)
+            st = cache.get(*req) if cache is not None else _hnc.structure_from_hnc(*req)
         elif p.structure == "file":
             if not p.structure_file:
                 raise abi.HydftError(abi.HY_ERR_INVALID, "structure = file needs structure_file")
@@ -270,6 +278,38 @@ class Model:
         abi.call("hy_model_advance", self.h, int(nsteps), int(dt_every), float(tend), pm,
                  buf.ctypes.data if buf is not None else None, C.byref(taken))
         return taken.value, (buf[:taken.value] if buf is not None else None)
+
+    # ---- captured steps (batched and served runs)
+    def graph_build(self, nsteps, dt_every=0, probe=None, ring=None):
+        """Capture nsteps RK3 steps (dt recomputed every dt_every) into a graph;
+        with `probe`, step s writes its probe row to ring slot (s-1) % ring."""
+        pm = (C.c_int * 3)(*[int(v) for v in probe]) if probe is not None else None
+        ring = int(ring or nsteps)
+        abi.call("hy_model_graph_build", self.h, int(nsteps), int(dt_every), pm, ring)
+        self._g_ring = ring
+
+    def graph_launch(self, nlaunch=1):
+        """Enqueue nlaunch replays on the model's stream (returns at once)."""
+        abi.call("hy_model_graph_launch", self.h, int(nlaunch))
+
+    def graph_probe(self):
+        """The probe ring: rows (t, Re, Im dn_k/n0, Re, Im mom_k, dt), slot (s-1) % ring."""
+        buf = np.zeros((self._g_ring, 6))
+        abi.call("hy_model_graph_probe", self.h, buf.ctypes.data)
+        return buf
+
+    def idle(self):
+        v = C.c_int()
+        abi.call("hy_model_idle", self.h, C.byref(v))
+        return bool(v.value)
+
+    def check(self):
+        abi.call("hy_model_check", self.h)
+
+    def device_bytes(self):
+        v = C.c_ulonglong()
+        abi.call("hy_model_device_bytes", self.h, C.byref(v))
+        return int(v.value)
 
     def rhs(self, t):
         dn = np.empty(self.shape)

@@ -1,0 +1,246 @@
+"""Batched and served chi(k, omega) and dispersion runs on one GPU.
+
+A request is one run: its grid, species, closure, time step, step count and
+either a drive (kind "chi": v_ext = A cos(k x - w t), the response
+chi = 2 <dn_k/n0 e^{i w t}> / A over the settled tail, analysis.f90:78-98) or a
+single-mode initial condition (kind "dispersion": frequency and damping of
+dn_k/n0, analysis.f90:21-74).
+
+`Server.serve` admits requests in arrival order.  A request takes a slot
+when fewer than `max_slots` runs are active and the active runs' device bytes
+plus its own stay within `hbm_budget`.  An admitted run has its own model on
+its own HIP stream.  Its steps are captured once as a graph of `chunk` steps,
+and all ceil(nsteps / chunk) replays are enqueued at once.  The device records
+every step's probe row in a ring.  The host loop then only polls the streams.
+A finished run is read and analysed and frees its slot; its device memory is
+released once no run is in flight, or earlier when the budget needs it
+(freeing device memory waits for all queued device work).  The next requests
+are admitted as slots and memory free up.  Runs overlap on the device while each
+run's own work keeps its stream order, so a run's result does not depend on
+what else runs beside it.
+
+`one_at_a_time` is the reference's way of running the same requests: one
+model at a time, steps taken by hy_model_advance calls.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .host import hnc as _hnc
+from .host.analysis import fit_damped_mode, fit_driven_response
+from .host.initial import set_initial_condition
+from .host.params import Params
+
+TWOPI = 2.0 * math.pi
+DT_EVERY = 20  # src/hydft.f90:42-43
+
+
+@dataclass
+class Request:
+    rid: int
+    p: Params
+    nsteps: int
+    kind: str = "chi"            # "chi" | "dispersion"
+    mode: tuple = (1, 0, 0)      # probed (and driven / excited) mode
+    arrival: float = 0.0         # seconds after the start of the stream
+    t_fit: float = -1.0          # analysis window start; < 0: the second half of the run
+
+
+@dataclass
+class Result:
+    rid: int
+    kind: str
+    arrival: float
+    start: float
+    done: float
+    steps: int
+    probe: np.ndarray = field(repr=False)
+    value: dict = field(default_factory=dict)
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+def estimate_bytes(p: Params, precision: str) -> int:
+    """Upper bound of a model's device memory, the terms of
+    hy_model_device_bytes from the grid alone (kept box of the 2/3 rule)."""
+    kx = p.nx // 3 + 1
+    ky = 2 * (p.ny // 3) + 1 if p.ny > 1 else 1
+    kz = 2 * (p.nz // 3) + 1 if p.nz > 1 else 1
+    nt = kx * ky * kz
+    esz = 8 if precision == "mixed" else 16
+    xs = 37 if p.closure == "maxwell" else 17
+    b = 16 * nt * 33 + esz * (kx * p.ny * p.nz * xs + kx * ky * p.nz * 24 + nt * 20)
+    b += 8 * (nt + max(p.ny * p.nz * 4, 4096)) + 8 * 6601 * 6 + (24 << 20)
+    return int(b)
+
+
+def make_model(req: Request, precision: str, cache=None):
+    """Species (c(k) from `cache`), model and initial state of a request."""
+    from .model import Model, build_species
+    p = req.p
+    sp = build_species(p, cache=cache)
+    m = Model(p, species=sp, precision=precision)
+    if req.kind == "chi":
+        m.set_state(np.full((p.nz, p.ny, p.nx), m.n0))
+    else:
+        m.set_state(set_initial_condition(p, m.n0, kind="mode", mode=req.mode))
+    return m
+
+
+def analyse(req: Request, rows: np.ndarray) -> dict:
+    t = rows[:, 0]
+    z = rows[:, 1] + 1j * rows[:, 2]
+    t0 = req.t_fit if req.t_fit >= 0.0 else 0.5 * t[-1]
+    if req.kind == "chi":
+        a, scatter = fit_driven_response(t, z, req.p.vext_omega, t0)
+        return {"chi": 2.0 * a / req.p.vext_amplitude, "scatter": scatter}
+    om, ga, nz = fit_damped_mode(t, z.real, t0)
+    return {"omega": om, "gamma": ga, "nzeros": nz}
+
+
+@dataclass
+class _Slot:
+    req: Request
+    model: object
+    start: float
+    bytes: int
+    steps: int
+
+
+class Server:
+    def __init__(self, hbm_budget: float = 200e9, max_slots: int = 64, chunk: int = DT_EVERY,
+                 precision: str = "mixed", cache=None):
+        self.hbm_budget = float(hbm_budget)
+        self.max_slots = int(max_slots)
+        self.chunk = int(chunk)
+        self.precision = precision
+        self.cache = cache if cache is not None else _hnc.StructureCache()
+        self.peak_active = 0
+        self.peak_bytes = 0
+        # finished models are destroyed only when no run is in flight (or when
+        # the budget needs their memory): freeing device memory waits for all
+        # queued work on the device and would stall the other runs' launches
+        self._retired = []
+
+    def _admit(self, req: Request, now: float) -> _Slot:
+        m = make_model(req, self.precision, self.cache)
+        launches = -(-req.nsteps // self.chunk)
+        steps = launches * self.chunk
+        m.graph_build(self.chunk, dt_every=DT_EVERY if req.p.dt <= 0.0 else 0, probe=req.mode, ring=steps)
+        m.graph_launch(launches)
+        # charged the larger of the estimate it was admitted on and its actual bytes
+        return _Slot(req, m, now, max(m.device_bytes(), estimate_bytes(req.p, self.precision)), steps)
+
+    def serve(self, requests, realtime: bool = True):
+        """Run the requests; with `realtime`, a request is admitted no earlier
+        than its arrival time.  Returns Results in completion order."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        active = []
+        results = []
+        used = 0
+        t0 = time.perf_counter()
+        while pending or active:
+            now = time.perf_counter() - t0
+            progressed = False
+            if not active and self._retired:
+                used -= self._release()
+            while pending and len(active) < self.max_slots and (not realtime or pending[0].arrival <= now):
+                need = estimate_bytes(pending[0].p, self.precision)
+                if active and used + need > self.hbm_budget and self._retired:
+                    used -= self._release()
+                if active and used + need > self.hbm_budget:
+                    break
+                req = pending.pop(0)
+                s = self._admit(req, now)
+                active.append(s)
+                used += s.bytes
+                self.peak_active = max(self.peak_active, len(active))
+                self.peak_bytes = max(self.peak_bytes, used)
+                progressed = True
+            for s in list(active):
+                if not s.model.idle():
+                    continue
+                s.model.check()
+                rows = s.model.graph_probe()[: s.req.nsteps]
+                done = time.perf_counter() - t0
+                results.append(Result(s.req.rid, s.req.kind, s.req.arrival, s.start, done, s.steps, rows,
+                                      analyse(s.req, rows)))
+                active.remove(s)
+                self._retired.append(s)
+                progressed = True
+            if not progressed:
+                time.sleep(0.0002)
+        used -= self._release()
+        return results
+
+    def _release(self) -> int:
+        freed = 0
+        for s in self._retired:
+            s.model.close()
+            freed += s.bytes
+        self._retired = []
+        return freed
+
+
+def one_at_a_time(requests, precision: str = "mixed", cache=None):
+    """The same requests, one model at a time, steps by hy_model_advance."""
+    cache = cache if cache is not None else _hnc.StructureCache()
+    results = []
+    t0 = time.perf_counter()
+    for req in sorted(requests, key=lambda r: r.arrival):
+        start = time.perf_counter() - t0
+        m = make_model(req, precision, cache)
+        taken, rows = m.advance(req.nsteps, dt_every=DT_EVERY if req.p.dt <= 0.0 else 0, probe=req.mode)
+        done = time.perf_counter() - t0
+        results.append(Result(req.rid, req.kind, req.arrival, start, done, taken, rows, analyse(req, rows)))
+        m.close()
+    return results
+
+
+def summary(results, wall: float) -> dict:
+    lat = np.array([r.latency for r in results])
+    return {"runs": len(results), "wall_s": wall, "runs_per_s": len(results) / wall,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99))}
+
+
+def chi_requests(n: int, seed: int = 7, grid=(32,), steps=400, arrival_rate=None, kind="ion") -> list:
+    """Driven-mode scan requests in the spirit of the batched configuration:
+    Yukawa ions, Gamma log-uniform in [1, 100], kappa uniform in [0.5, 3],
+    Newtonian/Maxwell alternating, drive 1e-4 cos(q x - w t) on mode 1 with
+    q uniform in [0.2, 2] (box L = 2 pi/q), w uniform in [0.1, 3] w_p
+    (w_p = 1 in these units: n0 = 3/(4 pi), e^2 = 1/3); CFL time step
+    recomputed every 20 steps, as the reference's driver does; seeded.
+    `arrival_rate`: Poisson arrivals at that rate (1/s), else all at t = 0."""
+    rng = np.random.default_rng(seed)
+    arr = np.random.default_rng([seed, 1])  # arrivals: their own stream (same runs with or without them)
+    out = []
+    t = 0.0
+    for i in range(n):
+        N = int(grid[i % len(grid)])
+        q = float(rng.uniform(0.2, 2.0))
+        p = Params()
+        p.kind = kind
+        p.gamma = float(np.exp(rng.uniform(np.log(1.0), np.log(100.0))))
+        p.kappa = float(rng.uniform(0.5, 3.0))
+        p.nx = p.ny = p.nz = N
+        p.lx = p.ly = p.lz = TWOPI / q
+        p.eta = 0.1
+        if i % 2:
+            p.closure, p.tau_model, p.tau = "maxwell", "constant", 1.0
+        else:
+            p.closure = "newtonian"
+        p.vext, p.vext_amplitude, p.vext_mode = "driven", 1.0e-4, (1, 0, 0)
+        p.vext_omega = float(rng.uniform(0.1, 3.0))
+        p.vext_ramp = 1.0
+        p.dt, p.cfl = 0.0, 0.4
+        p.finalize()
+        if arrival_rate:
+            t += float(arr.exponential(1.0 / arrival_rate))
+        out.append(Request(i, p, steps, "chi", (1, 0, 0), t))
+    return out

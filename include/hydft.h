@@ -174,6 +174,28 @@ int hy_model_profile(hy_model m, int enable);
 int hy_model_profile_read(hy_model m, int maxcls, int* launches, double* total_ms, double* bytes);
 const char* hy_profile_class_name(int cls);
 
+/* ---- batched and served runs: captured steps, stream state -------------
+   No counterpart in the reference (it runs one model at a time on the CPU).
+   hy_model_graph_build captures nsteps RK3 steps of the model as
+   hy_model_advance(m, nsteps, dt_every, 0, probe3, ...) takes them (dt
+   recomputed every dt_every steps, nsteps a multiple of it; no tend clamp)
+   into a HIP graph without advancing the model.  With probe3, every step
+   writes its probe row (6 doubles, as hy_model_advance) to slot
+   (step - 1) % ring of a device ring of `ring` >= nsteps rows, step being the
+   model's step count after the step.  hy_model_graph_launch enqueues replays
+   on the model's stream and returns at once; each replay advances the model by
+   nsteps.  hy_model_graph_probe synchronises and copies the ring (ring rows).
+   hy_model_idle reports whether the model's queued work has finished;
+   hy_model_check synchronises and returns the non-positive-density and
+   no-time-step conditions of the work so far.  hy_model_device_bytes is the
+   model's device memory (admission by HBM budget). */
+int hy_model_graph_build(hy_model m, int nsteps, int dt_every, const int* probe3, int ring);
+int hy_model_graph_launch(hy_model m, int nlaunch);
+int hy_model_graph_probe(hy_model m, double* h_probe);
+int hy_model_idle(hy_model m, int* idle);
+int hy_model_check(hy_model m);
+int hy_model_device_bytes(hy_model m, unsigned long long* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,85 @@
+"""Captured steps, batched and served runs on the GPU."""
+import numpy as np
+import pytest
+
+from hydft_amd import serve
+
+from . import _pairs as P
+
+
+def _req(rid, **kw):
+    r = serve.chi_requests(1, seed=100 + rid, grid=(kw.pop("N", 16),), steps=kw.pop("steps", 60))[0]
+    r.rid = rid
+    for k, v in kw.items():
+        setattr(r.p, k, v)
+    return r
+
+
+@pytest.mark.gpu
+def test_graph_replay_equals_advance(gpu):
+    """Replays of a captured 20-step sequence take exactly the steps of
+    hy_model_advance with the same dt cadence (FP64, bitwise)."""
+    req = _req(0, N=16)
+    cache = serve._hnc.StructureCache()
+    a = serve.make_model(req, "fp64", cache)
+    b = serve.make_model(req, "fp64", cache)
+    taken, rows_a = a.advance(60, dt_every=20, probe=req.mode)
+    assert taken == 60
+    b.graph_build(20, dt_every=20, probe=req.mode, ring=60)
+    b.graph_launch(3)
+    rows_b = b.graph_probe()
+    b.check()
+    na, ma, _ = a.get_state()
+    nb, mb, _ = b.get_state()
+    assert np.array_equal(na, nb) and np.array_equal(ma, mb)
+    assert np.array_equal(rows_a, rows_b)
+    assert a.time == b.time
+    # a further eager advance after the replays continues the same cadence
+    a.advance(20, dt_every=20)
+    b.advance(20, dt_every=20)
+    assert np.array_equal(a.get_state()[0], b.get_state()[0])
+    a.close(); b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_result_same_alone_and_in_the_mix(gpu, precision):
+    """Each request's probe series is the same served alone and served among
+    others of different grids and closures (both bitwise: the work of a run
+    keeps its own stream order)."""
+    reqs = [_req(i, N=(16, 24, 32)[i % 3]) for i in range(6)]
+    cache = serve._hnc.StructureCache()
+    alone = {}
+    for r in reqs[:3]:
+        res = serve.Server(precision=precision, cache=cache).serve([r], realtime=False)
+        alone[r.rid] = res[0]
+    srv = serve.Server(precision=precision, cache=cache, max_slots=6)
+    mix = {r.rid: r for r in srv.serve(reqs, realtime=False)}
+    assert srv.peak_active >= 4
+    for rid, ra in alone.items():
+        rb = mix[rid]
+        assert np.array_equal(ra.probe, rb.probe), rid
+        assert ra.value == rb.value or np.allclose(ra.value["chi"], rb.value["chi"], rtol=0, atol=0)
+
+
+@pytest.mark.gpu
+def test_admission_by_hbm_budget(gpu):
+    reqs = [_req(i, N=24, steps=40) for i in range(6)]
+    one = serve.estimate_bytes(reqs[0].p, "mixed")
+    srv = serve.Server(hbm_budget=2.5 * one, precision="mixed", max_slots=16)
+    out = srv.serve(reqs, realtime=False)
+    assert len(out) == 6
+    assert srv.peak_active <= 2
+    assert srv.peak_bytes <= 2.5 * one
+
+
+@pytest.mark.gpu
+def test_served_matches_one_at_a_time(gpu):
+    """Served results equal the one-model-at-a-time results of the same
+    requests (FP64, bitwise: same kernels, same order per run)."""
+    reqs = [_req(i, N=16, steps=40) for i in range(4)]
+    cache = serve._hnc.StructureCache()
+    a = {r.rid: r for r in serve.one_at_a_time(reqs, precision="fp64", cache=cache)}
+    b = {r.rid: r for r in serve.Server(precision="fp64", cache=cache).serve(reqs, realtime=False)}
+    for rid in a:
+        assert np.array_equal(a[rid].probe, b[rid].probe), rid
