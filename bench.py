@@ -199,27 +199,42 @@ def serve_sample(nreq, steps, grid=32):
     t0 = time.perf_counter()
     one = serve.one_at_a_time(reqs, precision="mixed", cache=cache)
     w_one = time.perf_counter() - t0
-    srv = serve.Server(precision="mixed", cache=cache, max_slots=64)
+    # concurrent runs, one stream and one captured graph per run
+    srv_st = serve.Server(precision="mixed", cache=cache, max_slots=64)
+    t0 = time.perf_counter()
+    streams = srv_st.serve(reqs, realtime=False)
+    w_streams = time.perf_counter() - t0
+    # batches: same-shape runs share every launch
+    srv = serve.Server(precision="mixed", cache=cache, max_slots=64, batch=True)
     t0 = time.perf_counter()
     batch = srv.serve(reqs, realtime=False)
     w_batch = time.perf_counter() - t0
-    # served stream: Poisson arrivals at 50 requests/s
-    reqs_rt = serve.chi_requests(nreq, seed=11, grid=(grid,), steps=steps, arrival_rate=50.0)
-    srv_rt = serve.Server(precision="mixed", cache=cache, max_slots=64)
+    # served stream: Poisson arrivals, batches formed from what has arrived
+    rate = 200.0
+    reqs_rt = serve.chi_requests(nreq, seed=11, grid=(grid,), steps=steps, arrival_rate=rate)
+    srv_rt = serve.Server(precision="mixed", cache=cache, max_slots=64, batch=True)
     t0 = time.perf_counter()
     served = srv_rt.serve(reqs_rt, realtime=True)
     w_served = time.perf_counter() - t0
-    same = all(np.array_equal(a.probe, b.probe) for a, b in
-               zip(sorted(one, key=lambda r: r.rid), sorted(batch, key=lambda r: r.rid)))
-    s_one, s_b, s_s = serve.summary(one, w_one), serve.summary(batch, w_batch), serve.summary(served, w_served)
+
+    def _same(xs):
+        return all(np.array_equal(a.probe, b.probe) for a, b in
+                   zip(sorted(one, key=lambda r: r.rid), sorted(xs, key=lambda r: r.rid)))
+
+    s_one, s_b = serve.summary(one, w_one), serve.summary(batch, w_batch)
+    s_st, s_s = serve.summary(streams, w_streams), serve.summary(served, w_served)
     return {"requests": nreq, "grid": grid, "steps_per_run": steps, "hnc_presolve_s": t_hnc,
-            "one_at_a_time_runs_per_s": s_one["runs_per_s"], "batched_runs_per_s": s_b["runs_per_s"],
+            "one_at_a_time_runs_per_s": s_one["runs_per_s"],
+            "streams_runs_per_s": s_st["runs_per_s"],
+            "batched_runs_per_s": s_b["runs_per_s"],
             "batched_over_one_at_a_time": s_b["runs_per_s"] / s_one["runs_per_s"],
-            "batched_peak_active": srv.peak_active,
-            "served_arrival_rate_per_s": 50.0, "served_runs_per_s": s_s["runs_per_s"],
+            "batched_launch_groups": srv.batches_launched,
+            "batched_latency_p50_s": s_b["latency_p50_s"], "batched_latency_p99_s": s_b["latency_p99_s"],
+            "served_arrival_rate_per_s": rate, "served_runs_per_s": s_s["runs_per_s"],
+            "served_launch_groups": srv_rt.batches_launched,
             "served_latency_p50_s": s_s["latency_p50_s"], "served_latency_p99_s": s_s["latency_p99_s"],
             "one_at_a_time_latency_p50_s": s_one["latency_p50_s"],
-            "batched_probe_series_equal_one_at_a_time": bool(same)}
+            "results_equal_one_at_a_time": bool(_same(batch) and _same(streams) and _same(served))}
 
 
 def main():
@@ -234,7 +249,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter runs")
-    ap.add_argument("--serve", type=int, default=32, metavar="NREQ",
+    ap.add_argument("--serve", type=int, default=64, metavar="NREQ",
                     help="batched/served chi(k, w) sample size (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=400)
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)

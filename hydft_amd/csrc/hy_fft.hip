@@ -221,8 +221,14 @@ __device__ inline int kept_index(int p, int h, int tailoff) {
 }
 
 // Inverse pass: in (A, K, B) stored spectrum (kept entries only) -> out (A, L, B).
+template <class T>
+struct PencilK {
+  PencilArgs a;
+  const cpx<T>* tw;
+};
+
 template <class T, int L>
-__global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_inv(PencilArgs a, const cpx<T>* __restrict__ twg) {
+__device__ __attribute__((always_inline)) inline void pencil_inv_body(const PencilArgs& a, const cpx<T>* __restrict__ twg) {
   using S = PencilShape<T, L>;
   constexpr int L1 = S::L1, L2 = S::L2, TPC = S::TPC, NA = S::NA;
   const int al = threadIdx.x % NA, g = threadIdx.x / NA;
@@ -290,7 +296,7 @@ __global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_inv(PencilAr
 
 // Forward pass: in (A, L, B) -> out (A, K, B) kept entries, scaled by 1/L.
 template <class T, int L>
-__global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_fwd(PencilArgs a, const cpx<T>* __restrict__ twg) {
+__device__ __attribute__((always_inline)) inline void pencil_fwd_body(const PencilArgs& a, const cpx<T>* __restrict__ twg) {
   using S = PencilShape<T, L>;
   constexpr int L1 = S::L1, L2 = S::L2, TPC = S::TPC, NA = S::NA;
   const int al = threadIdx.x % NA, g = threadIdx.x / NA;
@@ -351,17 +357,38 @@ __global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_fwd(PencilAr
 }
 
 template <class T, int L>
-static void launch_pencil(const PencilArgs& a, bool inverse, int nf, const cpx<T>* tw, hipStream_t s) {
+__global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_inv(PencilK<T> k) {
+  pencil_inv_body<T, L>(k.a, k.tw);
+}
+template <class T, int L>
+__global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_inv_b(const PencilK<T>* __restrict__ ks) {
+  pencil_inv_body<T, L>(ks[blockIdx.z].a, ks[blockIdx.z].tw);
+}
+template <class T, int L>
+__global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_fwd(PencilK<T> k) {
+  pencil_fwd_body<T, L>(k.a, k.tw);
+}
+template <class T, int L>
+__global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_fwd_b(const PencilK<T>* __restrict__ ks) {
+  pencil_fwd_body<T, L>(ks[blockIdx.z].a, ks[blockIdx.z].tw);
+}
+
+template <class T, int L>
+static int launch_pencil(const PencilArgs& a, bool inverse, int nf, const cpx<T>* tw, hipStream_t s) {
   using S = PencilShape<T, L>;
-  dim3 grid((unsigned)((a.A + S::NA - 1) / S::NA) * (unsigned)a.B * (unsigned)nf);
-  if (inverse) hipLaunchKernelGGL((k_pencil_inv<T, L>), grid, dim3(S::NT), 0, s, a, tw);
-  else hipLaunchKernelGGL((k_pencil_fwd<T, L>), grid, dim3(S::NT), 0, s, a, tw);
+  const dim3 grid((unsigned)((a.A + S::NA - 1) / S::NA) * (unsigned)a.B * (unsigned)nf);
+  const PencilK<T> k{a, tw};
+  if (inverse)
+    return launch(reinterpret_cast<const void*>(&k_pencil_inv<T, L>), reinterpret_cast<const void*>(&k_pencil_inv_b<T, L>),
+                  grid, dim3(S::NT), s, k);
+  return launch(reinterpret_cast<const void*>(&k_pencil_fwd<T, L>), reinterpret_cast<const void*>(&k_pencil_fwd_b<T, L>),
+                grid, dim3(S::NT), s, k);
 }
 
 template <class T>
 static int dispatch_pencil(const PencilArgs& a, bool inverse, int nf, const cpx<T>* tw, hipStream_t s) {
   switch (a.L) {
-#define HY_PL(LL) case LL: launch_pencil<T, LL>(a, inverse, nf, tw, s); break;
+#define HY_PL(LL) case LL: HY_TRY((launch_pencil<T, LL>(a, inverse, nf, tw, s))); break;
     HY_PL(4) HY_PL(8) HY_PL(12) HY_PL(16) HY_PL(24) HY_PL(32) HY_PL(48) HY_PL(64) HY_PL(96)
     HY_PL(128) HY_PL(256)
 #undef HY_PL
@@ -370,6 +397,11 @@ static int dispatch_pencil(const PencilArgs& a, bool inverse, int nf, const cpx<
   HY_HIP(hipGetLastError());
   HY_TRACE(inverse ? "pencil_inverse" : "pencil_forward", s);
   return HY_OK;
+}
+
+Recorder*& recorder() {
+  static thread_local Recorder* r = nullptr;
+  return r;
 }
 
 int pencil_pass(const Grid& g, int axis, bool inverse, bool trunc, int A, int B,

@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,62 @@ int trace_launch(const char* what, hipStream_t s);
     int r_ = (call);                \
     if (r_ != HY_OK) return r_;     \
   } while (0)
+
+// ------------------------------------------------------------ launches
+// Every kernel launch of the model's step path goes through hy::launch.  A
+// kernel K with argument struct A comes in two forms: K(A) (one model) and
+// K_b(const A*) (a batch: model blockIdx.z takes args[blockIdx.z]).  While a
+// recorder is active on the calling host thread, launch() appends the batched
+// form, the launch shape and the arguments to it instead of launching; a batch
+// of models with identical step sequences then replays each recorded launch
+// once for all of them.
+struct LaunchRec {
+  const void* fn_batched;
+  dim3 grid, block;
+  std::vector<unsigned char> args;
+  // x-line kernels: (argument block pointer, twiddle pointer) parameters; the
+  // batch gathers the blocks themselves into one device array
+  bool xline = false;
+  const void* xblock = nullptr;
+  const void* xtw = nullptr;
+};
+struct Recorder {
+  std::vector<LaunchRec> items;
+};
+Recorder*& recorder();  // the calling thread's active recorder (null: launch)
+
+template <class A>
+inline int launch(const void* fn_single, const void* fn_batched, dim3 grid, dim3 block, hipStream_t s,
+                  const A& a) {
+  if (Recorder* r = recorder()) {
+    LaunchRec it{fn_batched, grid, block, {}};
+    it.args.resize(sizeof(A));
+    std::memcpy(it.args.data(), &a, sizeof(A));
+    r->items.push_back(std::move(it));
+    return HY_OK;
+  }
+  void* args[] = {const_cast<A*>(&a)};
+  const hipError_t e = hipLaunchKernel(fn_single, grid, block, args, 0, s);
+  if (e != hipSuccess) return set_error(HY_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return HY_OK;
+}
+
+// x-line kernels: K(const Block* pa, const Tw* tw) / K_b(const Block* blocks, const Tw* tw)
+inline int launch_x(const void* fn_single, const void* fn_batched, dim3 grid, hipStream_t s, const void* pa,
+                    const void* tw) {
+  if (Recorder* r = recorder()) {
+    LaunchRec it{fn_batched, grid, dim3(64), {}};
+    it.xline = true;
+    it.xblock = pa;
+    it.xtw = tw;
+    r->items.push_back(std::move(it));
+    return HY_OK;
+  }
+  void* args[] = {const_cast<void**>(&pa), const_cast<void**>(&tw)};
+  const hipError_t e = hipLaunchKernel(fn_single, grid, dim3(64), args, 0, s);
+  if (e != hipSuccess) return set_error(HY_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return HY_OK;
+}
 
 // ---------------------------------------------------------------- complex
 template <class T>

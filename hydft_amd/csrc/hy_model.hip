@@ -109,7 +109,17 @@ static const double h_rk_a[3] = {32.0 / 60.0, 25.0 / 60.0, 45.0 / 60.0};
 static const double h_rk_b[3] = {0.0, -17.0 / 60.0, -25.0 / 60.0};
 
 enum CtlOp { CTL_STAGE_END = 0, CTL_STEP_END = 1, CTL_STEP_BEGIN = 2 };
-__global__ void k_ctl(Ctl* c, int op, int s, double tau, int evolve, double tend) {
+struct CtlK {
+  Ctl* c;
+  int op, s;
+  double tau;
+  int evolve;
+  double tend;
+};
+__device__ __attribute__((always_inline)) inline void ctl_body(const CtlK& k) {
+  Ctl* c = k.c;
+  const int op = k.op, s = k.s, evolve = k.evolve;
+  const double tau = k.tau, tend = k.tend;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (op == CTL_STEP_BEGIN) {
     if (tend > 0.0 && c->t + c->dt > tend) c->dt = tend - c->t;  // src/hydft.f90:44
@@ -129,6 +139,8 @@ __global__ void k_ctl(Ctl* c, int op, int s, double tau, int evolve, double tend
     c->ts = c->t;
   }
 }
+__global__ void k_ctl(CtlK k) { ctl_body(k); }
+__global__ void k_ctl_b(const CtlK* __restrict__ ks) { ctl_body(ks[blockIdx.z]); }
 
 // ------------------------------------------------------------ global memory
 // (Device compilation; a host-side build of this file, such as a CPU
@@ -430,7 +442,7 @@ struct PhysArgs {
   X::init(sm, twg, lane);
 
 template <class T, int NX, class X>
-__global__ __launch_bounds__(64) void k_phaseA(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
+__device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
   HY_XLINE_PREAMBLE
   // per-point values are held in the transform type T: in mixed precision
   // they come out of (and go back into) FP32 transforms; the pointwise
@@ -523,7 +535,7 @@ __global__ __launch_bounds__(64) void k_phaseA(const PhysArgs* __restrict__ pa, 
 }
 
 template <class T, int NX, class X, bool MX>
-__global__ __launch_bounds__(64) void k_phaseB(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
+__device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
   HY_XLINE_PREAMBLE
   const int iy = lok ? (int)(line % a.ny) : 0, iz = lok ? (int)(line / a.ny) : 0;
   T dn[PTS], gmx[PTS], gmy[PTS], gmz[PTS];
@@ -614,7 +626,7 @@ __global__ __launch_bounds__(64) void k_phaseB(const PhysArgs* __restrict__ pa, 
 
 // max |u_d| over a line (model.f90:171-172)
 template <class T, int NX, class X>
-__global__ __launch_bounds__(64) void k_umax(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
+__device__ __attribute__((always_inline)) inline void umax_body(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
   HY_XLINE_PREAMBLE
   T dn[PTS], m0[PTS], m1[PTS], m2[PTS];
 #pragma unroll
@@ -635,9 +647,48 @@ __global__ __launch_bounds__(64) void k_umax(const PhysArgs* __restrict__ pa, co
 }
 #undef HY_XLINE_PREAMBLE
 
+// The x kernels take the argument block pointer as a kernel parameter (one
+// model) or the base of a device array of blocks (a batch: model blockIdx.z).
+// A pointer that arrives inside a struct or is read from memory would lose
+// what the compiler knows about kernel parameters, and every field of the
+// block would then be loaded per lane instead of once per wave.
+template <class T, int NX, class X>
+__global__ __launch_bounds__(64) void k_phaseA(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ tw) {
+  phaseA_body<T, NX, X>(pa, tw);
+}
+template <class T, int NX, class X>
+__global__ __launch_bounds__(64) void k_phaseA_b(const PhysArgs* __restrict__ pas, const cpx<T>* __restrict__ tw) {
+  phaseA_body<T, NX, X>(pas + blockIdx.z, tw);
+}
+template <class T, int NX, class X, bool MX>
+__global__ __launch_bounds__(64) void k_phaseB(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ tw) {
+  phaseB_body<T, NX, X, MX>(pa, tw);
+}
+template <class T, int NX, class X, bool MX>
+__global__ __launch_bounds__(64) void k_phaseB_b(const PhysArgs* __restrict__ pas, const cpx<T>* __restrict__ tw) {
+  phaseB_body<T, NX, X, MX>(pas + blockIdx.z, tw);
+}
+template <class T, int NX, class X>
+__global__ __launch_bounds__(64) void k_umax(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ tw) {
+  umax_body<T, NX, X>(pa, tw);
+}
+template <class T, int NX, class X>
+__global__ __launch_bounds__(64) void k_umax_b(const PhysArgs* __restrict__ pas, const cpx<T>* __restrict__ tw) {
+  umax_body<T, NX, X>(pas + blockIdx.z, tw);
+}
+
 // compute_dt from the per-line maxima (model.f90:169-181)
-__global__ void k_dt(const double* __restrict__ red, int nlines, Ctl* c, double wmax, double numax,
-                     double dxmin, double cfl) {
+struct DtK {
+  const double* red;
+  int nlines;
+  Ctl* c;
+  double wmax, numax, dxmin, cfl;
+};
+__device__ __attribute__((always_inline)) inline void dt_body(const DtK& k) {
+  const double* __restrict__ red = k.red;
+  const int nlines = k.nlines;
+  Ctl* c = k.c;
+  const double wmax = k.wmax, numax = k.numax, dxmin = k.dxmin, cfl = k.cfl;
   __shared__ double sh[256];
   double m = 0.0;
   for (int i = threadIdx.x; i < nlines; i += blockDim.x) m = fmax(m, red[i]);
@@ -659,10 +710,21 @@ __global__ void k_dt(const double* __restrict__ red, int nlines, Ctl* c, double 
     else c->dt = cfl * dtm;
   }
 }
+__global__ void k_dt(DtK k) { dt_body(k); }
+__global__ void k_dt_b(const DtK* __restrict__ ks) { dt_body(ks[blockIdx.z]); }
 
 // sum over lines of `ncomp` partials, scaled: out[c] = scale * sum_l red[l*ncomp + c]
-__global__ void k_reduce_lines(const double* __restrict__ red, int nlines, int ncomp, double scale,
-                               double* __restrict__ out) {
+struct RedK {
+  const double* red;
+  int nlines, ncomp;
+  double scale;
+  double* out;
+};
+__device__ __attribute__((always_inline)) inline void reduce_lines_body(const RedK& k) {
+  const double* __restrict__ red = k.red;
+  const int nlines = k.nlines, ncomp = k.ncomp;
+  const double scale = k.scale;
+  double* __restrict__ out = k.out;
   __shared__ double sh[256];
   for (int c = 0; c < ncomp; ++c) {
     double s = 0.0;
@@ -677,6 +739,8 @@ __global__ void k_reduce_lines(const double* __restrict__ red, int nlines, int n
     __syncthreads();
   }
 }
+__global__ void k_reduce_lines(RedK k) { reduce_lines_body(k); }
+__global__ void k_reduce_lines_b(const RedK* __restrict__ ks) { reduce_lines_body(ks[blockIdx.z]); }
 
 // ------------------------------------------------------------ spectral kernels
 struct SpecArgs {
@@ -727,7 +791,7 @@ struct K1Args {
 };
 
 template <class T>
-__global__ void k_spec1(K1Args a) {
+__device__ __attribute__((always_inline)) inline void spec1_body(const K1Args& a) {
   const size_t nt = a.s.nt;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nt; e += (size_t)gridDim.x * blockDim.x) {
     c64 nh = a.N[e];
@@ -755,6 +819,10 @@ __global__ void k_spec1(K1Args a) {
     }
   }
 }
+template <class T>
+__global__ void k_spec1(K1Args a) { spec1_body<T>(a); }
+template <class T>
+__global__ void k_spec1_b(const K1Args* __restrict__ as) { spec1_body<T>(as[blockIdx.z]); }
 
 struct K2Args {
   SpecArgs s;
@@ -775,7 +843,7 @@ struct K2Args {
 // array index is a constant: a run-time index into the by-value K2Args (or
 // into a local array) would put them in per-lane scratch memory.
 template <class T, int ND>
-__global__ void k_spec2(K2Args a) {
+__device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a) {
   const size_t nt = a.s.nt;
   constexpr int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
   constexpr int idx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};  // stress.f90:92
@@ -856,6 +924,10 @@ __global__ void k_spec2(K2Args a) {
     }
   }
 }
+template <class T, int ND>
+__global__ void k_spec2(K2Args a) { spec2_body<T, ND>(a); }
+template <class T, int ND>
+__global__ void k_spec2_b(const K2Args* __restrict__ as) { spec2_body<T, ND>(as[blockIdx.z]); }
 
 // probe: spec_mode of n/n0 and mom_x (diagnostics.f90:70-77) after a step
 __global__ void k_probe(const Ctl* c, const c64* S, size_t nt, long long idx, double n0, double* out) {
@@ -871,8 +943,23 @@ __global__ void k_probe(const Ctl* c, const c64* S, size_t nt, long long idx, do
 // the same row into a ring of `ring` rows at slot (step - 1) % ring, where
 // step is the device step count after the step (captured steps: every replay
 // appends its rows)
-__global__ void k_probe_ring(const Ctl* c, const c64* S, size_t nt, long long idx, double n0, double* out,
-                             int ring) {
+struct ProbeK {
+  const Ctl* c;
+  const c64* S;
+  size_t nt;
+  long long idx;
+  double n0;
+  double* out;
+  int ring;
+};
+__device__ __attribute__((always_inline)) inline void probe_ring_body(const ProbeK& k) {
+  const Ctl* c = k.c;
+  const c64* S = k.S;
+  const size_t nt = k.nt;
+  const long long idx = k.idx;
+  const double n0 = k.n0;
+  double* out = k.out;
+  const int ring = k.ring;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const int slot = (int)(((long long)c->step - 1) % ring + ring) % ring;
   c64 a = idx >= 0 ? S[idx] : mk<double>(0, 0);
@@ -883,6 +970,8 @@ __global__ void k_probe_ring(const Ctl* c, const c64* S, size_t nt, long long id
   o[3] = b.x; o[4] = b.y;
   o[5] = c->dt;
 }
+__global__ void k_probe_ring(ProbeK k) { probe_ring_body(k); }
+__global__ void k_probe_ring_b(const ProbeK* __restrict__ ks) { probe_ring_body(ks[blockIdx.z]); }
 
 // full half spectrum <-> kept box
 __global__ void k_full_to_trunc(const c64* __restrict__ full, c64* __restrict__ tr, SpecArgs a, int nkx) {
@@ -1113,25 +1202,26 @@ static int fwd_chain(Model& m, const std::vector<std::pair<int, int>>& f /* (xsl
 enum XK { XK_A = 0, XK_B = 1, XK_UMAX = 2 };
 // device argument slots: one per distinct argument set
 enum XSlot { XA_STAGE = 0, XA_RELAX = 1, XB_STAGE = 2, XU_DT = 3, XA_N = 4 };
+#define HY_FN(K) reinterpret_cast<const void*>(&K)
 template <class T, int NX, class X, bool MX>
-static void launch_xk_with(int which, const PhysArgs* a, int nlines, const cpx<T>* tw, hipStream_t s) {
+static int launch_xk_with(int which, const PhysArgs* a, int nlines, const cpx<T>* tw, hipStream_t s) {
   const dim3 grid((unsigned)((nlines + X::LPW - 1) / X::LPW));
-  if (which == XK_A) hipLaunchKernelGGL((k_phaseA<T, NX, X>), grid, dim3(64), 0, s, a, tw);
-  else if (which == XK_B) hipLaunchKernelGGL((k_phaseB<T, NX, X, MX>), grid, dim3(64), 0, s, a, tw);
-  else hipLaunchKernelGGL((k_umax<T, NX, X>), grid, dim3(64), 0, s, a, tw);
+  if (which == XK_A) return launch_x(HY_FN((k_phaseA<T, NX, X>)), HY_FN((k_phaseA_b<T, NX, X>)), grid, s, a, tw);
+  if (which == XK_B) return launch_x(HY_FN((k_phaseB<T, NX, X, MX>)), HY_FN((k_phaseB_b<T, NX, X, MX>)), grid, s, a, tw);
+  return launch_x(HY_FN((k_umax<T, NX, X>)), HY_FN((k_umax_b<T, NX, X>)), grid, s, a, tw);
 }
 // phase B without the Maxwell advection code where the closure has no
 // evolving stress (fewer registers)
 template <class T, int NX>
-static void launch_xk(int which, bool maxwell, const PhysArgs* a, int nlines, const cpx<T>* tw, hipStream_t s) {
-  if (maxwell) launch_xk_with<T, NX, XWave<T, NX>, true>(which, a, nlines, tw, s);
-  else launch_xk_with<T, NX, XWave<T, NX>, false>(which, a, nlines, tw, s);
+static int launch_xk(int which, bool maxwell, const PhysArgs* a, int nlines, const cpx<T>* tw, hipStream_t s) {
+  if (maxwell) return launch_xk_with<T, NX, XWave<T, NX>, true>(which, a, nlines, tw, s);
+  return launch_xk_with<T, NX, XWave<T, NX>, false>(which, a, nlines, tw, s);
 }
 template <class T>
 static int dispatch_xk(int which, const Model& m, const PhysArgs* a, const cpx<T>* tw) {
   const int nlines = m.g.ny * m.g.nz;
   switch (m.g.nx) {
-#define HY_XK(LL) case LL: launch_xk<T, LL>(which, m.evolve != 0, a, nlines, tw, m.st); break;
+#define HY_XK(LL) case LL: HY_TRY((launch_xk<T, LL>(which, m.evolve != 0, a, nlines, tw, m.st))); break;
     HY_XK(4) HY_XK(8) HY_XK(12) HY_XK(16) HY_XK(24) HY_XK(32) HY_XK(48) HY_XK(64) HY_XK(96)
     HY_XK(128) HY_XK(256)
 #undef HY_XK
@@ -1170,13 +1260,14 @@ static double xk_bytes(const Model& m, int which, const PhysArgs& a) {
 // configuration, so each slot is uploaded on its first use, from pinned host
 // staging and in stream order.  Inside a graph capture the upload becomes a
 // copy node of the graph (its source stays valid and unchanged); a set that
-// changes while a capture is active is an error.
+// changes while a capture or a recording is active is an error.
 static int xargs_dev(Model& m, int slot, const PhysArgs& a, const PhysArgs** out) {
   const bool same = m.xargs_ok[slot] && std::memcmp(&m.p_xargs[slot], &a, sizeof(PhysArgs)) == 0;
   if (!same) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     HY_HIP(hipStreamIsCapturing(m.st, &cs));
-    if (cs == hipStreamCaptureStatusActive) {
+    if (cs == hipStreamCaptureStatusActive || recorder()) {
+      // a captured or recorded launch reads the slot as it is at replay
       if (m.xargs_ok[slot]) return set_error(HY_ERR_INVALID, "x-kernel arguments changed during graph capture");
     } else {
       HY_HIP(hipStreamSynchronize(m.st));  // no queued upload still reads the staging slot
@@ -1313,9 +1404,8 @@ static int stage(Model& m, int s, int rhs_only) {
   HY_TRY(run_xk(m, XK_B, XB_STAGE, b));
   HY_TRY(fwd_chain(m, fwdb));
   if (m.vext_kind != HY_VEXT_NONE) {
-    hipLaunchKernelGGL(k_reduce_lines, dim3(1), dim3(256), 0, m.st, m.red, g.ny * g.nz, 3,
-                       1.0 / (double)g.nreal(), m.ctl->nf);
-    HY_HIP(hipGetLastError());
+    const RedK rk{m.red, g.ny * g.nz, 3, 1.0 / (double)g.nreal(), m.ctl->nf};
+    HY_TRY(launch(HY_FN(k_reduce_lines), HY_FN(k_reduce_lines_b), dim3(1), dim3(256), m.st, rk));
   }
   // ---- K2: RHS assembly and the RK update
   HY_TRY(launch_k2(m, s, rhs_only));
@@ -1344,8 +1434,8 @@ static int launch_k1(Model& m, int relax) {
     if (m.evolve) b += nt * (3 * es + 3 * 16.0 * 2 + (relax ? 6 * 16.0 * 2 : 0.0));
     prof_begin(m, PC_K1, b);
   }
-  if (mixed(m)) hipLaunchKernelGGL(k_spec1<float>, dim3(nblocks(m.nt)), dim3(256), 0, m.st, a);
-  else hipLaunchKernelGGL(k_spec1<double>, dim3(nblocks(m.nt)), dim3(256), 0, m.st, a);
+  if (mixed(m)) HY_TRY(launch(HY_FN(k_spec1<float>), HY_FN(k_spec1_b<float>), dim3(nblocks(m.nt)), dim3(256), m.st, a));
+  else HY_TRY(launch(HY_FN(k_spec1<double>), HY_FN(k_spec1_b<double>), dim3(nblocks(m.nt)), dim3(256), m.st, a));
   prof_end(m);
   HY_HIP(hipGetLastError());
   HY_TRACE("spec_mu", m.st);
@@ -1380,12 +1470,14 @@ static int launch_k2(Model& m, int stage_idx, int rhs_only) {
   }
   const dim3 gr(nblocks(m.nt)), bl(256);
   switch (m.g.ndim * 2 + (mixed(m) ? 1 : 0)) {
-    case 2: hipLaunchKernelGGL((k_spec2<double, 1>), gr, bl, 0, m.st, a); break;
-    case 3: hipLaunchKernelGGL((k_spec2<float, 1>), gr, bl, 0, m.st, a); break;
-    case 4: hipLaunchKernelGGL((k_spec2<double, 2>), gr, bl, 0, m.st, a); break;
-    case 5: hipLaunchKernelGGL((k_spec2<float, 2>), gr, bl, 0, m.st, a); break;
-    case 6: hipLaunchKernelGGL((k_spec2<double, 3>), gr, bl, 0, m.st, a); break;
-    default: hipLaunchKernelGGL((k_spec2<float, 3>), gr, bl, 0, m.st, a); break;
+#define HY_K2(T, ND) HY_TRY(launch(HY_FN((k_spec2<T, ND>)), HY_FN((k_spec2_b<T, ND>)), gr, bl, m.st, a))
+    case 2: HY_K2(double, 1); break;
+    case 3: HY_K2(float, 1); break;
+    case 4: HY_K2(double, 2); break;
+    case 5: HY_K2(float, 2); break;
+    case 6: HY_K2(double, 3); break;
+    default: HY_K2(float, 3); break;
+#undef HY_K2
   }
   prof_end(m);
   HY_HIP(hipGetLastError());
@@ -1416,35 +1508,42 @@ static int finish_relax(Model& m) {
   return HY_OK;
 }
 
-__global__ void k_set_dt(Ctl* c, double dt) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) c->dt = dt;
+struct SetDtK {
+  Ctl* c;
+  double dt;
+};
+__device__ __attribute__((always_inline)) inline void set_dt_body(const SetDtK& k) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) k.c->dt = k.dt;
 }
+__global__ void k_set_dt(SetDtK k) { set_dt_body(k); }
+__global__ void k_set_dt_b(const SetDtK* __restrict__ ks) { set_dt_body(ks[blockIdx.z]); }
 
 static int compute_dt_dev(Model& m) {
   if (m.dt_fixed > 0.0) {  // model.f90:155-158
-    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, m.st, m.ctl, m.dt_fixed);
+    HY_TRY(launch(HY_FN(k_set_dt), HY_FN(k_set_dt_b), dim3(1), dim3(1), m.st, SetDtK{m.ctl, m.dt_fixed}));
     HY_HIP(hipGetLastError());
     return HY_OK;
   }
   PhysArgs a = phys_base(m);
   HY_TRY(stage_inputs(m, a, false));
   HY_TRY(run_xk(m, XK_UMAX, XU_DT, a));
-  hipLaunchKernelGGL(k_dt, dim3(1), dim3(256), 0, m.st, m.red, m.g.ny * m.g.nz, m.ctl, m.wmax,
-                     m.numax, m.dxmin, m.cfl);
-  HY_HIP(hipGetLastError());
-  return HY_OK;
+  const DtK dk{m.red, m.g.ny * m.g.nz, m.ctl, m.wmax, m.numax, m.dxmin, m.cfl};
+  return launch(HY_FN(k_dt), HY_FN(k_dt_b), dim3(1), dim3(256), m.st, dk);
+}
+
+static int ctl_op(Model& m, int op, int s, double tend) {
+  const CtlK k{m.ctl, op, s, m.tau, m.evolve, tend};
+  return launch(HY_FN(k_ctl), HY_FN(k_ctl_b), dim3(1), dim3(1), m.st, k);
 }
 
 static int step_once(Model& m, double tend) {
-  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, m.st, m.ctl, (int)CTL_STEP_BEGIN, 0, m.tau, m.evolve, tend);
+  HY_TRY(ctl_op(m, CTL_STEP_BEGIN, 0, tend));
   for (int s = 0; s < 3; ++s) {
     HY_TRY(stage(m, s, 0));
-    hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, m.st, m.ctl, (int)CTL_STAGE_END, s, m.tau, m.evolve, tend);
+    HY_TRY(ctl_op(m, CTL_STAGE_END, s, tend));
     if (m.evolve) m.relax_pending = 1;
   }
-  hipLaunchKernelGGL(k_ctl, dim3(1), dim3(1), 0, m.st, m.ctl, (int)CTL_STEP_END, 0, m.tau, m.evolve, tend);
-  HY_HIP(hipGetLastError());
-  return HY_OK;
+  return ctl_op(m, CTL_STEP_END, 0, tend);
 }
 
 static int read_ctl(Model& m, Ctl& c) {
@@ -2132,58 +2231,74 @@ int hy_model_advance(hy_model h, int nsteps, int dt_every, double tend, const in
 }
 
 // ---- captured steps
-int hy_model_graph_build(hy_model h, int nsteps, int dt_every, const int* probe3, int ring) {
-  if (!h || nsteps < 1 || dt_every < 0 || (probe3 && ring < nsteps))
-    return set_error(HY_ERR_INVALID, "hy_model_graph_build: bad argument");
-  Model& m = h->m;
+// Probe row index of mode (mx, my, mz) in the kept box, or -1 (not kept).
+static int probe_index(const Model& m, const int* probe3, long long* pidx) {
   const Grid& g = m.g;
-  if (dt_every > 0 && nsteps % dt_every != 0)
-    return set_error(HY_ERR_INVALID, "hy_model_graph_build: nsteps must be a multiple of dt_every");
-  long long pidx = -1;
-  if (probe3) {
-    int mx = probe3[0], my = probe3[1], mz = probe3[2];
-    if (mx < 0 || mx >= g.nkx) return set_error(HY_ERR_INVALID, "probe: mx out of range");
-    int wy = ((my % g.ny) + g.ny) % g.ny, wz = ((mz % g.nz) + g.nz) % g.nz;
-    int sy = wy <= g.ny / 2 ? wy : wy - g.ny, sz = wz <= g.nz / 2 ? wz : wz - g.nz;
-    if (mx < g.Kx && std::abs(sy) <= g.My && std::abs(sz) <= g.Mz) {
-      int ty = sy >= 0 ? sy : sy + g.Ky, tz = sz >= 0 ? sz : sz + g.Kz;
-      pidx = ((long long)tz * g.Ky + ty) * g.Kx + mx;
-    }
-    if (m.probe_cap < ring) {
-      HY_HIP(hipStreamSynchronize(m.st));
-      if (m.probe) (void)hipFree(m.probe);
-      m.probe = nullptr;
-      m.probe_cap = 0;
-      HY_TRY(dmalloc((void**)&m.probe, sizeof(double) * 6 * ring));
-      m.probe_cap = ring;
-    }
+  *pidx = -1;
+  if (!probe3) return HY_OK;
+  int mx = probe3[0], my = probe3[1], mz = probe3[2];
+  if (mx < 0 || mx >= g.nkx) return set_error(HY_ERR_INVALID, "probe: mx out of range");
+  int wy = ((my % g.ny) + g.ny) % g.ny, wz = ((mz % g.nz) + g.nz) % g.nz;
+  int sy = wy <= g.ny / 2 ? wy : wy - g.ny, sz = wz <= g.nz / 2 ? wz : wz - g.nz;
+  if (mx < g.Kx && std::abs(sy) <= g.My && std::abs(sz) <= g.Mz) {
+    int ty = sy >= 0 ? sy : sy + g.Ky, tz = sz >= 0 ? sz : sz + g.Kz;
+    *pidx = ((long long)tz * g.Ky + ty) * g.Kx + mx;
   }
-  // the captured sequence starts and ends with no pending relaxation and at
-  // the start of the dt cadence, so replays chain like hy_model_advance calls
+  return HY_OK;
+}
+
+// Make the model ready to emit a replayable sequence of nsteps steps: probe
+// ring allocated, no pending relaxation, start of the dt cadence, stream idle.
+static int prepare_steps(Model& m, int nsteps, int dt_every, const int* probe3, int ring) {
+  if (nsteps < 1 || dt_every < 0 || (probe3 && ring < nsteps)) return set_error(HY_ERR_INVALID, "bad step sequence");
+  if (dt_every > 0 && nsteps % dt_every != 0)
+    return set_error(HY_ERR_INVALID, "nsteps must be a multiple of dt_every");
+  if (probe3 && m.probe_cap < ring) {
+    HY_HIP(hipStreamSynchronize(m.st));
+    if (m.probe) (void)hipFree(m.probe);
+    m.probe = nullptr;
+    m.probe_cap = 0;
+    HY_TRY(dmalloc((void**)&m.probe, sizeof(double) * 6 * ring));
+    m.probe_cap = ring;
+  }
   HY_TRY(finish_relax(m));
   if (dt_every > 0) m.irecompute = 0;
   HY_HIP(hipStreamSynchronize(m.st));
+  return HY_OK;
+}
+
+// The launches of nsteps steps as hy_model_advance(nsteps, dt_every, tend=0)
+// takes them, each step followed by its probe row into the ring; ends with
+// no pending relaxation and at the start of the dt cadence again.
+static int emit_steps(Model& m, int nsteps, int dt_every, long long pidx, bool probe, int ring) {
+  for (int i = 0; i < nsteps; ++i) {
+    if (dt_every > 0) {
+      if (m.irecompute == 0) {
+        HY_TRY(finish_relax(m));
+        HY_TRY(compute_dt_dev(m));
+      }
+      m.irecompute = (m.irecompute + 1) % dt_every;
+    }
+    HY_TRY(step_once(m, 0.0));
+    if (probe)
+      HY_TRY(launch(HY_FN(k_probe_ring), HY_FN(k_probe_ring_b), dim3(1), dim3(1), m.st,
+                    ProbeK{m.ctl, m.S, m.nt, pidx, m.n0, m.probe, ring}));
+  }
+  return finish_relax(m);
+}
+
+int hy_model_graph_build(hy_model h, int nsteps, int dt_every, const int* probe3, int ring) {
+  if (!h) return set_error(HY_ERR_INVALID, "hy_model_graph_build: null argument");
+  Model& m = h->m;
+  long long pidx = -1;
+  HY_TRY(probe_index(m, probe3, &pidx));
+  HY_TRY(prepare_steps(m, nsteps, dt_every, probe3, ring));
   if (m.gexec) { (void)hipGraphExecDestroy(m.gexec); m.gexec = nullptr; }
   if (m.graph) { (void)hipGraphDestroy(m.graph); m.graph = nullptr; }
   const bool prof = m.prof;
   m.prof = false;
   HY_HIP(hipStreamBeginCapture(m.st, hipStreamCaptureModeThreadLocal));
-  int rc = HY_OK;
-  for (int i = 0; i < nsteps && rc == HY_OK; ++i) {
-    if (dt_every > 0) {
-      if (m.irecompute == 0) {
-        rc = finish_relax(m);
-        if (rc == HY_OK) rc = compute_dt_dev(m);
-      }
-      m.irecompute = (m.irecompute + 1) % dt_every;
-    }
-    if (rc == HY_OK) rc = step_once(m, 0.0);
-    if (rc == HY_OK && probe3) {
-      hipLaunchKernelGGL(k_probe_ring, dim3(1), dim3(1), 0, m.st, m.ctl, m.S, m.nt, pidx, m.n0, m.probe, ring);
-      if (hipGetLastError() != hipSuccess) rc = set_error(HY_ERR_HIP, "probe launch failed during capture");
-    }
-  }
-  if (rc == HY_OK) rc = finish_relax(m);
+  const int rc = emit_steps(m, nsteps, dt_every, pidx, probe3 != nullptr, ring);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(m.st, &graph);
   m.prof = prof;
@@ -2248,6 +2363,159 @@ int hy_model_device_bytes(hy_model h, unsigned long long* bytes) {
   if (m.d_real) b += sizeof(double) * (double)g.nreal();
   if (m.d_full) b += sizeof(c64) * (double)g.nfull();
   *bytes = (unsigned long long)b;
+  return HY_OK;
+}
+
+// ---- batches: one launch per kernel for many models of the same shape
+struct hy_batch_s {
+  std::vector<hy_model> models;
+  hipStream_t st = nullptr;
+  struct Item {
+    const void* fn;
+    dim3 grid, block;
+    void* d_args;      // B argument structs, or B argument blocks (x-line kernels)
+    bool xline;
+    const void* tw;
+  };
+  std::vector<Item> items;
+  void* d_all = nullptr;
+};
+
+static void batch_free(hy_batch_s* b) {
+  if (!b) return;
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  if (b->d_all) (void)hipFree(b->d_all);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+}
+
+int hy_batch_create(const hy_model* ms, int n, int nsteps, int dt_every, const int* probe3, int ring,
+                    hy_batch* out) {
+  if (!ms || !out || n < 1) return set_error(HY_ERR_INVALID, "hy_batch_create: bad argument");
+  int maxz = 0;
+  HY_HIP(hipDeviceGetAttribute(&maxz, hipDeviceAttributeMaxGridDimZ, 0));
+  if (n > maxz) return set_error(HY_ERR_INVALID, "hy_batch_create: too many models for one launch");
+  std::vector<Recorder> recs(n);
+  for (int i = 0; i < n; ++i) {
+    if (!ms[i]) return set_error(HY_ERR_INVALID, "hy_batch_create: null model");
+    Model& m = ms[i]->m;
+    long long pidx = -1;
+    HY_TRY(probe_index(m, probe3, &pidx));
+    HY_TRY(prepare_steps(m, nsteps, dt_every, probe3, ring));
+    const bool prof = m.prof;
+    m.prof = false;
+    recorder() = &recs[i];
+    const int rc = emit_steps(m, nsteps, dt_every, pidx, probe3 != nullptr, ring);
+    recorder() = nullptr;
+    m.prof = prof;
+    HY_TRY(rc);
+    HY_HIP(hipStreamSynchronize(m.st));  // argument-block uploads done during the recording
+  }
+  // every model must have produced the same sequence of launches
+  const size_t L = recs[0].items.size();
+  for (int i = 1; i < n; ++i) {
+    if (recs[i].items.size() != L) return set_error(HY_ERR_INVALID, "hy_batch_create: models differ in shape or options");
+    for (size_t p = 0; p < L; ++p) {
+      const LaunchRec& a = recs[0].items[p];
+      const LaunchRec& c = recs[i].items[p];
+      if (a.fn_batched != c.fn_batched || a.grid.x != c.grid.x || a.grid.y != c.grid.y || a.block.x != c.block.x ||
+          a.xline != c.xline || a.args.size() != c.args.size())
+        return set_error(HY_ERR_INVALID, "hy_batch_create: models differ in shape or options");
+    }
+  }
+  // one device allocation: per launch position, n argument structs (or blocks)
+  auto* b = new hy_batch_s();
+  b->models.assign(ms, ms + n);
+  std::vector<size_t> off(L);
+  size_t total = 0;
+  for (size_t p = 0; p < L; ++p) {
+    off[p] = total;
+    const size_t each = recs[0].items[p].xline ? sizeof(PhysArgs) : recs[0].items[p].args.size();
+    total += (each * n + 255) & ~(size_t)255;
+  }
+  std::vector<unsigned char> host(total, 0);
+  for (size_t p = 0; p < L; ++p) {
+    const bool xl = recs[0].items[p].xline;
+    for (int i = 0; i < n; ++i) {
+      const LaunchRec& it = recs[i].items[p];
+      if (xl) {
+        const Model& m = ms[i]->m;
+        const PhysArgs* blk = reinterpret_cast<const PhysArgs*>(it.xblock);
+        const long slot = (long)(blk - m.d_xargs);
+        if (slot < 0 || slot >= XA_N) {
+          batch_free(b);
+          return set_error(HY_ERR_INVALID, "hy_batch_create: unknown x-kernel argument block");
+        }
+        std::memcpy(&host[off[p] + sizeof(PhysArgs) * i], &m.p_xargs[slot], sizeof(PhysArgs));
+      } else {
+        std::memcpy(&host[off[p] + it.args.size() * i], it.args.data(), it.args.size());
+      }
+    }
+  }
+  if (hipMalloc(&b->d_all, std::max<size_t>(total, 256)) != hipSuccess) {
+    (void)hipGetLastError();
+    b->d_all = nullptr;
+    batch_free(b);
+    return set_error(HY_ERR_OOM, "hy_batch_create: device allocation failed");
+  }
+  if (hipMemcpy(b->d_all, host.data(), total, hipMemcpyHostToDevice) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess) {
+    batch_free(b);
+    return set_error(HY_ERR_HIP, "hy_batch_create: upload failed");
+  }
+  for (size_t p = 0; p < L; ++p) {
+    const LaunchRec& it = recs[0].items[p];
+    // the twiddle table of a line length is the same in every model of the batch
+    b->items.push_back({it.fn_batched, dim3(it.grid.x, it.grid.y, (unsigned)n), it.block,
+                        (char*)b->d_all + off[p], it.xline, it.xtw});
+  }
+  *out = b;
+  return HY_OK;
+}
+
+int hy_batch_launch(hy_batch b, int nlaunch) {
+  if (!b || nlaunch < 0) return set_error(HY_ERR_INVALID, "hy_batch_launch: bad argument");
+  for (hy_model h : b->models) HY_HIP(hipStreamSynchronize(h->m.st));  // the models' own queued work first
+  for (int l = 0; l < nlaunch; ++l)
+    for (auto& it : b->items) {
+      hipError_t e;
+      if (it.xline) {
+        void* args[] = {&it.d_args, const_cast<void**>(&it.tw)};
+        e = hipLaunchKernel(it.fn, it.grid, it.block, args, 0, b->st);
+      } else {
+        void* args[] = {&it.d_args};
+        e = hipLaunchKernel(it.fn, it.grid, it.block, args, 0, b->st);
+      }
+      if (e != hipSuccess) return set_error(HY_ERR_HIP, std::string("hy_batch_launch: ") + hipGetErrorString(e));
+    }
+  return HY_OK;
+}
+
+int hy_batch_idle(hy_batch b, int* idle) {
+  if (!b || !idle) return set_error(HY_ERR_INVALID, "hy_batch_idle: null argument");
+  const hipError_t e = hipStreamQuery(b->st);
+  if (e == hipSuccess) { *idle = 1; return HY_OK; }
+  if (e == hipErrorNotReady) { (void)hipGetLastError(); *idle = 0; return HY_OK; }
+  return set_error(HY_ERR_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
+int hy_batch_sync(hy_batch b) {
+  if (!b) return set_error(HY_ERR_INVALID, "null argument");
+  HY_HIP(hipStreamSynchronize(b->st));
+  return HY_OK;
+}
+
+int hy_batch_destroy(hy_batch b) {
+  batch_free(b);
+  return HY_OK;
+}
+
+int hy_model_probe_ring(hy_model h, int ring, double* h_probe) {
+  if (!h || !h_probe || ring < 1) return set_error(HY_ERR_INVALID, "hy_model_probe_ring: bad argument");
+  Model& m = h->m;
+  if (ring > m.probe_cap) return set_error(HY_ERR_INVALID, "hy_model_probe_ring: ring larger than the model's");
+  HY_HIP(hipStreamSynchronize(m.st));
+  HY_HIP(hipMemcpy(h_probe, m.probe, sizeof(double) * 6 * ring, hipMemcpyDeviceToHost));
   return HY_OK;
 }
 

@@ -311,6 +311,12 @@ class Model:
         abi.call("hy_model_device_bytes", self.h, C.byref(v))
         return int(v.value)
 
+    def probe_ring(self, ring):
+        """`ring` rows of the device probe ring (slot (s-1) % ring for step s)."""
+        buf = np.zeros((int(ring), 6))
+        abi.call("hy_model_probe_ring", self.h, int(ring), buf.ctypes.data)
+        return buf
+
     def rhs(self, t):
         dn = np.empty(self.shape)
         dmom = np.empty((3,) + self.shape)
@@ -339,3 +345,45 @@ class Model:
         out = C.c_double()
         abi.call("hy_model_kernel", self.h, float(k), C.byref(out))
         return out.value
+
+
+class Batch:
+    """Same-shape models stepped together: each kernel of a step is launched
+    once for all of them (hy_batch_*).  The models keep their own state and
+    probe rings; do not use them while the batch is in flight."""
+
+    def __init__(self, models, nsteps, dt_every=0, probe=None, ring=None):
+        self.models = list(models)
+        hs = (C.c_void_p * len(self.models))(*[m.h for m in self.models])
+        pm = (C.c_int * 3)(*[int(v) for v in probe]) if probe is not None else None
+        self.ring = int(ring or nsteps)
+        self.nsteps = int(nsteps)
+        h = C.c_void_p()
+        abi.call("hy_batch_create", hs, len(self.models), int(nsteps), int(dt_every), pm, self.ring,
+                 C.byref(h))
+        self.h = h.value
+
+    def launch(self, nlaunch=1):
+        abi.call("hy_batch_launch", self.h, int(nlaunch))
+
+    def idle(self):
+        v = C.c_int()
+        abi.call("hy_batch_idle", self.h, C.byref(v))
+        return bool(v.value)
+
+    def sync(self):
+        abi.call("hy_batch_sync", self.h)
+
+    def probe(self, i):
+        return self.models[i].probe_ring(self.ring)
+
+    def close(self):
+        if getattr(self, "h", None):
+            abi.call("hy_batch_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -19,6 +19,16 @@ are admitted as slots and memory free up.  Runs overlap on the device while each
 run's own work keeps its stream order, so a run's result does not depend on
 what else runs beside it.
 
+With `batch=True` the server admits the arrived requests in groups of the
+same shape: same grid, species kind, closure, drive kind, probed mode, time
+step policy and step count.  A group runs as one batch (hy_batch_*): each
+kernel of a step is launched once for all its models (grid z = model), so a
+step of B small runs costs the launches of one.  A batch is one slot; its
+models' device bytes are charged to the budget as a whole.  A group the
+library refuses as a batch runs as separate captured runs.  Batching changes
+how the kernels are launched, not what they compute: each run's result is
+bitwise the result it has alone.
+
 `one_at_a_time` is the reference's way of running the same requests: one
 model at a time, steps taken by hy_model_advance calls.
 """
@@ -106,36 +116,99 @@ def analyse(req: Request, rows: np.ndarray) -> dict:
 
 @dataclass
 class _Slot:
-    req: Request
-    model: object
+    reqs: list
+    models: list
     start: float
     bytes: int
     steps: int
+    batch: object = None
+
+    def idle(self):
+        return self.batch.idle() if self.batch is not None else self.models[0].idle()
+
+    def rows(self, i):
+        m = self.models[i]
+        return self.batch.probe(i) if self.batch is not None else m.graph_probe()
+
+    def close(self):
+        if self.batch is not None:
+            self.batch.close()
+        for m in self.models:
+            m.close()
+
+
+def batch_key(req: Request) -> tuple:
+    """Requests with equal keys run the same kernel sequence."""
+    p = req.p
+    return (p.nx, p.ny, p.nz, p.kind, p.closure, p.tau_model, p.vext, p.dt <= 0.0, req.nsteps,
+            tuple(req.mode), req.kind)
 
 
 class Server:
     def __init__(self, hbm_budget: float = 200e9, max_slots: int = 64, chunk: int = DT_EVERY,
-                 precision: str = "mixed", cache=None):
+                 precision: str = "mixed", cache=None, batch: bool = False, max_batch: int = 256):
         self.hbm_budget = float(hbm_budget)
         self.max_slots = int(max_slots)
         self.chunk = int(chunk)
         self.precision = precision
         self.cache = cache if cache is not None else _hnc.StructureCache()
-        self.peak_active = 0
+        self.batch = bool(batch)
+        self.max_batch = int(max_batch)
+        self.peak_active = 0      # runs in flight
         self.peak_bytes = 0
+        self.batches_launched = 0  # slots: batches and single runs
         # finished models are destroyed only when no run is in flight (or when
         # the budget needs their memory): freeing device memory waits for all
         # queued work on the device and would stall the other runs' launches
         self._retired = []
 
-    def _admit(self, req: Request, now: float) -> _Slot:
-        m = make_model(req, self.precision, self.cache)
+    def _admit(self, reqs, now: float) -> list:
+        """Start `reqs` (one batch key) as one batch, or each as a captured run."""
+        req = reqs[0]
         launches = -(-req.nsteps // self.chunk)
         steps = launches * self.chunk
-        m.graph_build(self.chunk, dt_every=DT_EVERY if req.p.dt <= 0.0 else 0, probe=req.mode, ring=steps)
-        m.graph_launch(launches)
-        # charged the larger of the estimate it was admitted on and its actual bytes
-        return _Slot(req, m, now, max(m.device_bytes(), estimate_bytes(req.p, self.precision)), steps)
+        dt_every = DT_EVERY if req.p.dt <= 0.0 else 0
+        models = [make_model(r, self.precision, self.cache) for r in reqs]
+        # each run is charged the larger of the estimate it was admitted on and its actual bytes
+        charge = [max(m.device_bytes(), estimate_bytes(r.p, self.precision)) for m, r in zip(models, reqs)]
+        if len(reqs) > 1:
+            from .model import Batch
+            from . import abi
+            try:
+                b = Batch(models, self.chunk, dt_every=dt_every, probe=req.mode, ring=steps)
+            except abi.HydftError as e:
+                if e.code != abi.HY_ERR_INVALID:
+                    raise
+            else:
+                b.launch(launches)
+                self.batches_launched += 1
+                return [_Slot(list(reqs), models, now, sum(charge), steps, b)]
+        out = []
+        for r, m, c in zip(reqs, models, charge):
+            m.graph_build(self.chunk, dt_every=dt_every, probe=r.mode, ring=steps)
+            m.graph_launch(launches)
+            self.batches_launched += 1
+            out.append(_Slot([r], [m], now, c, steps))
+        return out
+
+    def _take(self, pending, now, realtime, room_bytes):
+        """The next group to admit: the first arrived request and, when
+        batching, the arrived requests of its key, within the budget."""
+        head = pending[0]
+        group = [head]
+        if self.batch:
+            key = batch_key(head)
+            need = estimate_bytes(head.p, self.precision)
+            for r in pending[1:]:
+                if realtime and r.arrival > now:
+                    break
+                if len(group) >= self.max_batch or need * (len(group) + 1) > room_bytes:
+                    break
+                if batch_key(r) == key:
+                    group.append(r)
+        for r in group:
+            pending.remove(r)
+        return group
 
     def serve(self, requests, realtime: bool = True):
         """Run the requests; with `realtime`, a request is admitted no earlier
@@ -156,21 +229,23 @@ class Server:
                     used -= self._release()
                 if active and used + need > self.hbm_budget:
                     break
-                req = pending.pop(0)
-                s = self._admit(req, now)
-                active.append(s)
-                used += s.bytes
-                self.peak_active = max(self.peak_active, len(active))
+                room = self.hbm_budget - used if active else max(self.hbm_budget - used, need)
+                group = self._take(pending, now, realtime, room)
+                for s in self._admit(group, now):
+                    active.append(s)
+                    used += s.bytes
+                self.peak_active = max(self.peak_active, sum(len(s.models) for s in active))
                 self.peak_bytes = max(self.peak_bytes, used)
                 progressed = True
             for s in list(active):
-                if not s.model.idle():
+                if not s.idle():
                     continue
-                s.model.check()
-                rows = s.model.graph_probe()[: s.req.nsteps]
-                done = time.perf_counter() - t0
-                results.append(Result(s.req.rid, s.req.kind, s.req.arrival, s.start, done, s.steps, rows,
-                                      analyse(s.req, rows)))
+                for i, (req, m) in enumerate(zip(s.reqs, s.models)):
+                    m.check()
+                    rows = s.rows(i)[: req.nsteps]
+                    done = time.perf_counter() - t0
+                    results.append(Result(req.rid, req.kind, req.arrival, s.start, done, s.steps, rows,
+                                          analyse(req, rows)))
                 active.remove(s)
                 self._retired.append(s)
                 progressed = True
@@ -182,7 +257,7 @@ class Server:
     def _release(self) -> int:
         freed = 0
         for s in self._retired:
-            s.model.close()
+            s.close()
             freed += s.bytes
         self._retired = []
         return freed

@@ -196,6 +196,28 @@ int hy_model_idle(hy_model m, int* idle);
 int hy_model_check(hy_model m);
 int hy_model_device_bytes(hy_model m, unsigned long long* bytes);
 
+/* ---- batches of same-shape models: one launch per kernel for all ---------
+   No counterpart in the reference.  For many small runs the device's launch
+   rate, not its bandwidth, bounds a step (a step is ~45 launches); a batch
+   launches each kernel of the step once for all its models (grid z = model).
+   hy_batch_create records nsteps steps of each model as hy_model_graph_build
+   would capture them (same arguments and preconditions) and checks that every
+   model yields the same kernels and grids (same grid, precision, closure and
+   options); the models' own state, tables and probe rings are used in place.
+   hy_batch_launch enqueues nlaunch passes of those nsteps steps on the
+   batch's stream and returns; hy_batch_idle / hy_batch_sync poll / wait.
+   While a batch is in flight its models must not be used otherwise; a model
+   stays usable after hy_batch_destroy (its step count and time then read the
+   device's).  hy_model_probe_ring copies `ring` probe rows of a model. */
+typedef struct hy_batch_s* hy_batch;
+int hy_batch_create(const hy_model* models, int n, int nsteps, int dt_every, const int* probe3, int ring,
+                    hy_batch* out);
+int hy_batch_launch(hy_batch b, int nlaunch);
+int hy_batch_idle(hy_batch b, int* idle);
+int hy_batch_sync(hy_batch b);
+int hy_batch_destroy(hy_batch b);
+int hy_model_probe_ring(hy_model m, int ring, double* h_probe);
+
 #ifdef __cplusplus
 }
 #endif

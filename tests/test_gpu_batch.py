@@ -1,0 +1,102 @@
+"""Batches of same-shape runs: one launch per kernel for all models
+(hy_batch_*).  A model's result in a batch is the result it has alone."""
+import numpy as np
+import pytest
+
+from hydft_amd import abi, serve
+from hydft_amd.model import Batch
+
+
+def _reqs(n, N=16, steps=40, closure=None, seed=300):
+    out = []
+    for r in serve.chi_requests(3 * n, seed=seed, grid=(N,), steps=steps):
+        if closure is None or r.p.closure == closure:
+            out.append(r)
+    return out[:n]
+
+
+def _alone(reqs, precision, cache, steps, dt_every):
+    res = []
+    for r in reqs:
+        m = serve.make_model(r, precision, cache)
+        taken, rows = m.advance(steps, dt_every=dt_every, probe=r.mode)
+        assert taken == steps
+        res.append((m.get_state(), rows, m.time))
+        m.close()
+    return res
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,closure", [("fp64", "newtonian"), ("fp64", "maxwell"),
+                                               ("mixed", "newtonian"), ("mixed", "maxwell")])
+def test_batch_equals_alone(gpu, precision, closure):
+    """Every model of a batch ends bitwise where it ends alone: state,
+    probe series (CFL dt recomputed every 20 steps) and time."""
+    reqs = _reqs(5, closure=closure)
+    cache = serve._hnc.StructureCache()
+    alone = _alone(reqs, precision, cache, 40, 20)
+    ms = [serve.make_model(r, precision, cache) for r in reqs]
+    b = Batch(ms, 20, dt_every=20, probe=(1, 0, 0), ring=40)
+    b.launch(2)
+    b.sync()
+    for i, m in enumerate(ms):
+        (na, ma, pa), rows, t = alone[i]
+        nb, mb, pb = m.get_state()
+        assert np.array_equal(na, nb) and np.array_equal(ma, mb), i
+        if pa is not None:
+            assert np.array_equal(pa, pb), i
+        assert np.array_equal(rows, b.probe(i)), i
+        assert m.time == t, i
+    b.close()
+    for m in ms:
+        m.close()
+
+
+@pytest.mark.gpu
+def test_batch_models_usable_after(gpu):
+    """After the batch is destroyed its models continue with hy_model_advance
+    exactly as a model that took all steps alone."""
+    reqs = _reqs(3, closure="newtonian", seed=310)
+    cache = serve._hnc.StructureCache()
+    alone = _alone(reqs, "fp64", cache, 60, 20)
+    ms = [serve.make_model(r, "fp64", cache) for r in reqs]
+    b = Batch(ms, 20, dt_every=20)
+    b.launch(2)
+    b.close()
+    for i, m in enumerate(ms):
+        m.advance(20, dt_every=20)
+        assert np.array_equal(m.get_state()[0], alone[i][0][0]), i
+        assert m.time == alone[i][2]
+        m.close()
+
+
+@pytest.mark.gpu
+def test_batch_rejects_mixed_shapes(gpu):
+    cache = serve._hnc.StructureCache()
+    a = _reqs(1, N=16, closure="newtonian")[0]
+    c = _reqs(1, N=24, closure="newtonian")[0]
+    d = _reqs(1, N=16, closure="maxwell")[0]
+    for other in (c, d):
+        ms = [serve.make_model(a, "fp64", cache), serve.make_model(other, "fp64", cache)]
+        with pytest.raises(abi.HydftError) as e:
+            Batch(ms, 20, dt_every=20)
+        assert e.value.code == abi.HY_ERR_INVALID
+        for m in ms:
+            m.close()
+
+
+@pytest.mark.gpu
+def test_batched_server_matches_alone(gpu):
+    """The batching server groups same-shape requests; each result equals
+    the one-at-a-time result (FP64, bitwise)."""
+    reqs = [serve.chi_requests(1, seed=400 + i, grid=((16, 24)[i % 2],), steps=40)[0] for i in range(8)]
+    for i, r in enumerate(reqs):
+        r.rid = i
+    cache = serve._hnc.StructureCache()
+    a = {r.rid: r for r in serve.one_at_a_time(reqs, precision="fp64", cache=cache)}
+    srv = serve.Server(precision="fp64", cache=cache, batch=True)
+    b = {r.rid: r for r in srv.serve(reqs, realtime=False)}
+    assert srv.batches_launched < len(reqs)
+    for rid in a:
+        assert np.array_equal(a[rid].probe, b[rid].probe), rid
+        assert a[rid].value["chi"] == b[rid].value["chi"], rid
