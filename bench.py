@@ -183,15 +183,17 @@ def kernel_traffic(tr, cls, precision):
     return None
 
 
-def serve_sample(nreq, steps, grid=32):
+def serve_sample(nreq, steps, grids=(32, 48, 64)):
     """Batched / served chi(k, w) throughput against one run at a time, on a
-    seeded sample of driven-mode requests (Yukawa ions, Gamma in [1, 100],
-    kappa in [0.5, 3], Newtonian / Maxwell alternating, CFL dt), FP32-mixed.
-    HNC c(k) are solved (in parallel host processes) before the timed parts."""
+    seeded sample of driven-mode requests in the form of the batched scan:
+    Yukawa ions, grids cycling through 32^3, 48^3, 64^3, Gamma log-uniform in
+    [1, 100], kappa in [0.5, 3], Newtonian / Maxwell alternating, drive
+    1e-4 cos(q x - w t), CFL dt; FP32-mixed.  HNC c(k) are solved (in parallel
+    host processes) before the timed parts."""
     from hydft_amd import serve
     from hydft_amd.host import hnc
     from hydft_amd.model import hnc_request
-    reqs = serve.chi_requests(nreq, seed=11, grid=(grid,), steps=steps)
+    reqs = serve.chi_requests(nreq, seed=11, grid=grids, steps=steps)
     cache = hnc.StructureCache()
     t0 = time.perf_counter()
     cache.solve_many([hnc_request(r.p) for r in reqs])
@@ -210,8 +212,8 @@ def serve_sample(nreq, steps, grid=32):
     batch = srv.serve(reqs, realtime=False)
     w_batch = time.perf_counter() - t0
     # served stream: Poisson arrivals, batches formed from what has arrived
-    rate = 200.0
-    reqs_rt = serve.chi_requests(nreq, seed=11, grid=(grid,), steps=steps, arrival_rate=rate)
+    rate = 50.0
+    reqs_rt = serve.chi_requests(nreq, seed=11, grid=grids, steps=steps, arrival_rate=rate)
     srv_rt = serve.Server(precision="mixed", cache=cache, max_slots=64, batch=True)
     t0 = time.perf_counter()
     served = srv_rt.serve(reqs_rt, realtime=True)
@@ -223,7 +225,7 @@ def serve_sample(nreq, steps, grid=32):
 
     s_one, s_b = serve.summary(one, w_one), serve.summary(batch, w_batch)
     s_st, s_s = serve.summary(streams, w_streams), serve.summary(served, w_served)
-    return {"requests": nreq, "grid": grid, "steps_per_run": steps, "hnc_presolve_s": t_hnc,
+    return {"requests": nreq, "grids": list(grids), "steps_per_run": steps, "hnc_presolve_s": t_hnc,
             "one_at_a_time_runs_per_s": s_one["runs_per_s"],
             "streams_runs_per_s": s_st["runs_per_s"],
             "batched_runs_per_s": s_b["runs_per_s"],
@@ -249,7 +251,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter runs")
-    ap.add_argument("--serve", type=int, default=64, metavar="NREQ",
+    ap.add_argument("--serve", type=int, default=96, metavar="NREQ",
                     help="batched/served chi(k, w) sample size (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=400)
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)

@@ -408,7 +408,7 @@ struct PhysArgs {
   double dx, dy, dz;
   const double* kx;
   double n0, mass, beta, cn, alpha0, cg;
-  double dadn;  // d alpha / d n at n0: start of the per-point Newton iteration
+  double dadn, d2adn2;  // d alpha / d n, d2 alpha / d n2 at n0: start of the per-point Newton iteration
   int ideal, tf, grad;
   const double* fdtab;
   Ctl* ctl;
@@ -482,7 +482,8 @@ __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs
     double v = a.ideal ? log1p((double)dn[p] / a.n0) / a.beta : 0.0;  // fe_ideal_classical.f90:40
     if (a.tf || a.grad) {
       const double al = live && n > 0.0
-                            ? fd::alpha_from_ip12_dev(a.fdtab, n / a.cn, a.alpha0 + a.dadn * (double)dn[p])
+                            ? fd::alpha_from_ip12_dev(a.fdtab, n / a.cn,
+                                                      a.alpha0 + (double)dn[p] * (a.dadn + 0.5 * a.d2adn2 * (double)dn[p]))
                             : a.alpha0;
       if (a.tf) v += (al - a.alpha0) / a.beta;  // fe_kinetic_tfk.f90:80-81
       if (a.grad) {                             // fe_kinetic_tfk.f90:130-137
@@ -1004,6 +1005,7 @@ struct Model {
   int ideal = 0, tf = 0, grad = 0, hartree = 0, ry = 0;
   double beta = 3, cn = 0, alpha0 = 0, cg = 0, e2 = 1.0 / 3.0, kappa = 0;
   double dadn = 0;  // 1 / (dn/dalpha) at alpha0: 2 / (cn I_{-1/2}(alpha0))
+  double d2adn2 = 0;  // d2 alpha / dn2 at alpha0: -dadn^2 I'_{-1/2} / I_{-1/2}
   std::vector<hy_term_desc> terms;
   std::vector<std::vector<double>> ktabs, ctabs;
   int closure = 0, evolve = 0, stress_newt = 0;
@@ -1297,6 +1299,7 @@ static PhysArgs phys_base(Model& m) {
   a.dx = g.dx; a.dy = g.dy; a.dz = g.dz; a.kx = g.d_kx;
   a.n0 = m.n0; a.mass = m.mass; a.beta = m.beta; a.cn = m.cn; a.alpha0 = m.alpha0; a.cg = m.cg;
   a.dadn = m.dadn;
+  a.d2adn2 = m.d2adn2;
   a.ideal = m.ideal; a.tf = m.tf; a.grad = m.grad;
   a.fdtab = m.fdtab; a.ctl = m.ctl; a.evolve = m.evolve;
   a.vext_kind = m.vext_kind; a.vext_td = m.vext_td; a.amp = m.vamp; a.ramp = m.vramp;
@@ -1964,6 +1967,7 @@ int model_build(Model& m, const hy_model_desc* d) {
     HY_TRY(dmalloc((void**)&m.fdtab, sizeof(double) * fd::NTAB * fd::NC));
     HY_HIP(hipMemcpy(m.fdtab, tab, sizeof(double) * fd::NTAB * fd::NC, hipMemcpyHostToDevice));
     m.dadn = 2.0 / (m.cn * fd::im12(tab, m.alpha0));
+    m.d2adn2 = -m.dadn * m.dadn * fd::dim12(tab, m.alpha0) / fd::im12(tab, m.alpha0);
   }
   HY_TRY(dmalloc((void**)&m.d_xargs, sizeof(PhysArgs) * XA_N));
   if (hipHostMalloc((void**)&m.p_xargs, sizeof(PhysArgs) * XA_N, hipHostMallocDefault) != hipSuccess) {
