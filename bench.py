@@ -239,6 +239,34 @@ def serve_sample(nreq, steps, grids=(32, 48, 64)):
             "results_equal_one_at_a_time": bool(_same(batch) and _same(streams) and _same(served))}
 
 
+def batch_only(spec, steps):
+    """Step one batch of same-shape driven-mode runs (spec "B,N,closure") and
+    print model-steps/s: the batched kernels alone, for profiling."""
+    from hydft_amd import abi, serve
+    from hydft_amd.host import hnc
+    from hydft_amd.model import Batch, hnc_request
+    b_s, n_s, clos = spec.split(",")
+    nb, ng = int(b_s), int(n_s)
+    abi.call("hy_init", 0)
+    reqs = [r for r in serve.chi_requests(4 * nb, seed=11, grid=(ng,), steps=steps) if r.p.closure == clos][:nb]
+    cache = hnc.StructureCache()
+    cache.solve_many([hnc_request(r.p) for r in reqs])
+    ms = [serve.make_model(r, "mixed", cache) for r in reqs]
+    b = Batch(ms, 20, dt_every=20, probe=(1, 0, 0), ring=steps)
+    b.launch(1)
+    b.sync()
+    t0 = time.perf_counter()
+    b.launch(steps // 20 - 1)
+    b.sync()
+    w = time.perf_counter() - t0
+    print(json.dumps({"batch": nb, "grid": ng, "closure": clos, "steps": steps - 20,
+                      "batch_steps_per_s": (steps - 20) / w, "model_steps_per_s": nb * (steps - 20) / w}))
+    b.close()
+    for m in ms:
+        m.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -254,10 +282,14 @@ def main():
     ap.add_argument("--serve", type=int, default=96, metavar="NREQ",
                     help="batched/served chi(k, w) sample size (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=400)
+    ap.add_argument("--batch-only", default="", metavar="B,N,CLOSURE",
+                    help="only step one batch of B driven-mode runs on N^3 (profiling)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
         return pmc_child(args)
+    if args.batch_only:
+        return batch_only(args.batch_only, args.serve_steps)
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; runs on more than one GPU are not part of it "
               "(independent runs would shard with no exchange).", file=sys.stderr)

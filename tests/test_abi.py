@@ -28,3 +28,35 @@ def test_host_entry_points_without_gpu():
     except abi.HydftError as e:
         assert e.code == abi.HY_ERR_INVALID and "y > 0" in str(e)
     assert abi.lib().lib.hy_profile_class_name(0) == b"pencil_z_inverse"
+
+
+def test_fortran_descriptors_match_c_layout(tmp_path):
+    """The bind(C) mirrors in hydft_hip_model.f90 have the size and field
+    offsets of the C structs (checked against the ctypes mirror)."""
+    import os
+    import shutil
+    import subprocess
+    import pytest
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    mod = os.path.join(os.path.dirname(abi.__file__), "_obj", "fmod")
+    if not os.path.exists(fc) or not os.path.exists(os.path.join(mod, "hydft_hip_model.mod")):
+        pytest.skip("amdflang or the built Fortran module is missing")
+    src = tmp_path / "lay.f90"
+    src.write_text("""program lay
+  use, intrinsic :: iso_c_binding
+  use hydft_hip_model
+  type(hy_term_desc), target :: t
+  type(hy_model_desc), target :: d
+  integer(c_intptr_t) :: b
+  b = transfer(c_loc(d), b)
+  print '(6(i0,1x))', c_sizeof(t), c_sizeof(d), transfer(c_loc(d%terms), b) - b, &
+    transfer(c_loc(d%vext_mode), b) - b, transfer(c_loc(d%cfl), b) - b, &
+    transfer(c_loc(t%c_tab), b) - transfer(c_loc(t), b)
+end program
+""")
+    exe = tmp_path / "lay"
+    subprocess.run([fc, "-I", mod, str(src), "-o", str(exe)], check=True, capture_output=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = [C.sizeof(abi.TermDesc), C.sizeof(abi.ModelDesc), abi.ModelDesc.terms.offset,
+            abi.ModelDesc.vext_mode.offset, abi.ModelDesc.cfl.offset, abi.TermDesc.c_tab.offset]
+    assert got == want
