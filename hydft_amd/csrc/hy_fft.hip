@@ -202,14 +202,14 @@ __device__ inline cpx<T> load_any(const void* p, size_t i, int is64) {
   return cvt<T>(reinterpret_cast<const c32*>(p)[i]);
 }
 
-// Block shape of the pencil kernels: NA contiguous columns (128 bytes of one
-// row per quarter wave) times TPC threads per column; columns of length
+// Block shape of the pencil kernels: NA contiguous columns (256 bytes of one
+// row in FP32, 128 in FP64) times TPC threads per column; columns of length
 // <= 32 take one thread each, 64 per block.
 template <class T, int L>
 struct PencilShape {
   static constexpr ColPlan P = col_plan(L, (int)sizeof(cpx<T>));
   static constexpr int L1 = P.L1, L2 = P.L2, TPC = P.TPC;
-  static constexpr int NA = TPC == 1 ? 64 : (sizeof(T) == 4 ? 16 : 8);
+  static constexpr int NA = TPC == 1 ? 64 : (sizeof(T) == 4 ? 32 : 8);
   static constexpr int NT = NA * TPC;
   static constexpr int J1 = L2 / TPC;  // step-1 DFTs (length L1) per thread
   static constexpr int J2 = L1 / TPC;  // step-2 DFTs (length L2) per thread

@@ -116,19 +116,21 @@ __device__ inline void rdft_combine(cpx<T> (&s)[R][M], cpx<T>* v) {
 }
 
 // ------------------------------------------------------ two-level column plans
-// (L1, L2, TPC) per length and element size; L1 * L2 = L, TPC divides both.
-// TPC = 1 keeps a whole column in one thread's registers (short columns; FP64
-// columns longer than 16 are split so that the thread fits in 128 VGPRs).
+// (L1, L2, TPC) per length; L1 * L2 = L, TPC divides both.  TPC = 1 keeps a
+// whole column in one thread's registers (columns of at most 16 points);
+// longer columns are split over TPC threads with one LDS transpose, which
+// keeps a thread well under 128 VGPRs (a 32-point FP32 column in one thread
+// needs about 140 and runs at 3 waves per SIMD).
 struct ColPlan { int L1, L2, TPC; };
-__host__ __device__ constexpr ColPlan col_plan(int L, int elem_bytes) {
+__host__ __device__ constexpr ColPlan col_plan(int L, int /*elem_bytes*/) {
   return L == 256 ? ColPlan{16, 16, 16}
        : L == 128 ? ColPlan{16, 8, 8}
        : L == 96  ? ColPlan{12, 8, 4}
        : L == 64  ? ColPlan{8, 8, 8}
        : L == 48  ? ColPlan{12, 4, 4}
-       : (L == 32 && elem_bytes > 8) ? ColPlan{8, 4, 4}
-       : (L == 24 && elem_bytes > 8) ? ColPlan{6, 4, 2}
-                                      : ColPlan{L, 1, 1};
+       : L == 32  ? ColPlan{8, 4, 4}
+       : L == 24  ? ColPlan{6, 4, 2}
+                  : ColPlan{L, 1, 1};
 }
 
 }  // namespace hy
