@@ -205,15 +205,18 @@ struct XIn {
 enum XMul { XM_NONE = 0, XM_IK = 1, XM_NEGK2 = 2 };
 
 template <class T>
-__device__ inline cpx<T> xin_load(const XIn& in, size_t line, int Kx, int kk, const double* kx) {
+__device__ inline cpx<T> xin_load(const XIn& in, size_t line, int Kx, int kk, double dkx) {
   const size_t e = line * Kx + kk;
   cpx<T> v = in.is64 ? cvt<T>(gldc<double>(in.p, e)) : gldc<T>(in.p, e);
   if (in.k0off != 0.0 && kk == 0 && line == 0) v.x = (T)((double)v.x + in.k0off);
+  // kx of column kk is kk * dkx, computed here: a table load would be a
+  // generic-address load whose wait also drains the wave's pending loads
   if (in.mul == XM_IK) {
-    const T k = (T)kx[kk];
+    const T k = (T)((double)kk * dkx);
     v = mk<T>(-k * v.y, k * v.x);
   } else if (in.mul == XM_NEGK2) {
-    const T k2 = (T)(kx[kk] * kx[kk]);
+    const double kd = (double)kk * dkx;
+    const T k2 = (T)(kd * kd);
     v = mk<T>(-k2 * v.x, -k2 * v.y);
   }
   return v;
@@ -308,7 +311,7 @@ __device__ inline void xfft(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, int lane, 
 // b(x) are returned in qa[q], qb[q] (qb may be null).
 template <class T, int NX>
 __device__ inline void x_inv_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, const XIn* A, const XIn* B,
-                                  size_t line, int Kx, const double* kx, int lane, T* qa, T* qb) {
+                                  size_t line, int Kx, double dkx, int lane, T* qa, T* qb) {
 #pragma unroll
   for (int q = 0; q < XQ<NX>::Q; ++q) {  // slots of idle lanes stay defined
     qa[q] = 0;
@@ -320,8 +323,8 @@ __device__ inline void x_inv_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, cons
     const int kk = hi ? NX - e : e;
     cpx<T> a = mk<T>(0, 0), b = mk<T>(0, 0);
     if (kk < Kx) {
-      if (A) a = xin_load<T>(*A, line, Kx, kk, kx);
-      if (B) b = xin_load<T>(*B, line, Kx, kk, kx);
+      if (A) a = xin_load<T>(*A, line, Kx, kk, dkx);
+      if (B) b = xin_load<T>(*B, line, Kx, kk, dkx);
     }
     if (kk == 0 || 2 * kk == NX) { a.y = 0; b.y = 0; }
     return hi ? mk<T>(a.x + b.y, -a.y + b.x)   // conj(a) + i conj(b)
@@ -383,8 +386,8 @@ struct XWave {
   }
   __device__ static int xpos(int g, int q) { return xown<NX>(g, q); }
   __device__ static void inv_pair(Smem& s, const XIn* A, const XIn* B, size_t line, bool, int Kx,
-                                  const double* kx, int g, int, T* qa, T* qb) {
-    x_inv_pair<T, NX>(s.b0, s.b1, s.tw, A, B, line, Kx, kx, g, qa, qb);
+                                  double dkx, int g, int, T* qa, T* qb) {
+    x_inv_pair<T, NX>(s.b0, s.b1, s.tw, A, B, line, Kx, dkx, g, qa, qb);
   }
   template <class F>
   __device__ static void fwd_pair(Smem& s, F&& val, void* oA, void* oB, size_t line, bool, int Kx, int g, int) {
@@ -406,7 +409,7 @@ enum OutB { OB_H1 = 0, OB_H2, OB_H3, OB_ADV1 };  // ADV1..ADV6 = 3..8
 struct PhysArgs {
   int nd, Kx, ny, nz;
   double dx, dy, dz;
-  const double* kx;
+  double dkx;  // 2 pi / lx: kx of column kk is kk * dkx
   double n0, mass, beta, cn, alpha0, cg;
   double dadn, d2adn2;  // d alpha / d n, d2 alpha / d n2 at n0: start of the per-point Newton iteration
   int ideal, tf, grad;
@@ -453,20 +456,20 @@ __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { m0[p] = 0; g2[p] = 0; lap[p] = 0; }
   if (a.grad) {
-    X::inv_pair(sm, &a.n, &a.lap, line, lok, a.Kx, a.kx, g, lw, dn, lap);
+    X::inv_pair(sm, &a.n, &a.lap, line, lok, a.Kx, a.dkx, g, lw, dn, lap);
     T gx[PTS], gy[PTS];
 #pragma unroll
     for (int p = 0; p < PTS; ++p) gy[p] = 0;
-    X::inv_pair(sm, &a.gx, nd >= 2 ? &a.gy : nullptr, line, lok, a.Kx, a.kx, g, lw, gx, gy);
+    X::inv_pair(sm, &a.gx, nd >= 2 ? &a.gy : nullptr, line, lok, a.Kx, a.dkx, g, lw, gx, gy);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) g2[p] = gx[p] * gx[p] + gy[p] * gy[p];
 #pragma unroll
     for (int p = 0; p < PTS; ++p) gx[p] = 0;
-    X::inv_pair(sm, nd >= 3 ? &a.gz : nullptr, &a.m[0], line, lok, a.Kx, a.kx, g, lw, gx, m0);
+    X::inv_pair(sm, nd >= 3 ? &a.gz : nullptr, &a.m[0], line, lok, a.Kx, a.dkx, g, lw, gx, m0);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) g2[p] += gx[p] * gx[p];
   } else {
-    X::inv_pair(sm, &a.n, &a.m[0], line, lok, a.Kx, a.kx, g, lw, dn, m0);
+    X::inv_pair(sm, &a.n, &a.m[0], line, lok, a.Kx, a.dkx, g, lw, dn, m0);
   }
   // pointwise functional in FP64
   T mu[PTS], inv[PTS];  // mu_loc, 1 / (m n)
@@ -502,7 +505,7 @@ __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs
   T m1[PTS], m2[PTS];
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { m1[p] = 0; m2[p] = 0; }
-  if (nd >= 2) X::inv_pair(sm, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, line, lok, a.Kx, a.kx, g, lw, m1, m2);
+  if (nd >= 2) X::inv_pair(sm, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, line, lok, a.Kx, a.dkx, g, lw, m1, m2);
   // outputs, one uniform switch per output (not per point)
   auto fill = [&](int c, T* v) {
 #pragma unroll
@@ -542,8 +545,8 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
   T dn[PTS], gmx[PTS], gmy[PTS], gmz[PTS];
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { gmy[p] = 0; gmz[p] = 0; }
-  X::inv_pair(sm, &a.n, &a.mux, line, lok, a.Kx, a.kx, g, lw, dn, gmx);
-  if (nd >= 2) X::inv_pair(sm, &a.muy, nd >= 3 ? &a.muz : nullptr, line, lok, a.Kx, a.kx, g, lw, gmy, gmz);
+  X::inv_pair(sm, &a.n, &a.mux, line, lok, a.Kx, a.dkx, g, lw, dn, gmx);
+  if (nd >= 2) X::inv_pair(sm, &a.muy, nd >= 3 ? &a.muz : nullptr, line, lok, a.Kx, a.dkx, g, lw, gmy, gmz);
   // external force (external_potential.f90:65-93) at the stage-start time,
   // evaluated point by point
   const bool wave_f = a.vext_kind == HY_VEXT_MODE || a.vext_kind == HY_VEXT_DRIVEN;
@@ -594,8 +597,8 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
     T m0[PTS], m1[PTS], m2[PTS];
 #pragma unroll
     for (int p = 0; p < PTS; ++p) { m1[p] = 0; m2[p] = 0; }
-    X::inv_pair(sm, &a.m[0], nd >= 2 ? &a.m[1] : nullptr, line, lok, a.Kx, a.kx, g, lw, m0, m1);
-    if (nd >= 3) X::inv_pair(sm, &a.m[2], nullptr, line, lok, a.Kx, a.kx, g, lw, m2, nullptr);
+    X::inv_pair(sm, &a.m[0], nd >= 2 ? &a.m[1] : nullptr, line, lok, a.Kx, a.dkx, g, lw, m0, m1);
+    if (nd >= 3) X::inv_pair(sm, &a.m[2], nullptr, line, lok, a.Kx, a.dkx, g, lw, m2, nullptr);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) {
       const double inv = 1.0 / (a.mass * (a.n0 + dn[p]));
@@ -607,11 +610,11 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
 #pragma unroll
     for (int p = 0; p < PTS; ++p) { ya[p] = 0; za[p] = 0; yb[p] = 0; zb[p] = 0; }
     if (nd == 1) {
-      X::inv_pair(sm, &a.px[c], &a.px[c + 1], line, lok, a.Kx, a.kx, g, lw, xa, xb);
+      X::inv_pair(sm, &a.px[c], &a.px[c + 1], line, lok, a.Kx, a.dkx, g, lw, xa, xb);
     } else {
-      X::inv_pair(sm, &a.px[c], &a.py[c], line, lok, a.Kx, a.kx, g, lw, xa, ya);
-      X::inv_pair(sm, &a.px[c + 1], &a.py[c + 1], line, lok, a.Kx, a.kx, g, lw, xb, yb);
-      if (nd >= 3) X::inv_pair(sm, &a.pz[c], &a.pz[c + 1], line, lok, a.Kx, a.kx, g, lw, za, zb);
+      X::inv_pair(sm, &a.px[c], &a.py[c], line, lok, a.Kx, a.dkx, g, lw, xa, ya);
+      X::inv_pair(sm, &a.px[c + 1], &a.py[c + 1], line, lok, a.Kx, a.dkx, g, lw, xb, yb);
+      if (nd >= 3) X::inv_pair(sm, &a.pz[c], &a.pz[c + 1], line, lok, a.Kx, a.dkx, g, lw, za, zb);
     }
     X::fwd_pair(
         sm,
@@ -632,8 +635,8 @@ __device__ __attribute__((always_inline)) inline void umax_body(const PhysArgs* 
   T dn[PTS], m0[PTS], m1[PTS], m2[PTS];
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { m1[p] = 0; m2[p] = 0; }
-  X::inv_pair(sm, &a.n, &a.m[0], line, lok, a.Kx, a.kx, g, lw, dn, m0);
-  if (nd >= 2) X::inv_pair(sm, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, line, lok, a.Kx, a.kx, g, lw, m1, m2);
+  X::inv_pair(sm, &a.n, &a.m[0], line, lok, a.Kx, a.dkx, g, lw, dn, m0);
+  if (nd >= 2) X::inv_pair(sm, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, line, lok, a.Kx, a.dkx, g, lw, m1, m2);
   double um = 0.0;
 #pragma unroll
   for (int p = 0; p < PTS; ++p) {
@@ -1296,7 +1299,7 @@ static PhysArgs phys_base(Model& m) {
   std::memset(&a, 0, sizeof(a));
   const Grid& g = m.g;
   a.nd = g.ndim; a.Kx = g.Kx; a.ny = g.ny; a.nz = g.nz;
-  a.dx = g.dx; a.dy = g.dy; a.dz = g.dz; a.kx = g.d_kx;
+  a.dx = g.dx; a.dy = g.dy; a.dz = g.dz; a.dkx = 2.0 * fd::PI / g.lx;
   a.n0 = m.n0; a.mass = m.mass; a.beta = m.beta; a.cn = m.cn; a.alpha0 = m.alpha0; a.cg = m.cg;
   a.dadn = m.dadn;
   a.d2adn2 = m.d2adn2;
