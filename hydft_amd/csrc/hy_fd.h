@@ -192,13 +192,16 @@ __host__ __device__ inline double alpha_from_ip12_from(const double* tab, double
 }
 
 // ---------------------------------------------------------------- device batch
-// Device-side table access through the global address space with 32-bit
-// offsets (scalar base + one vector offset per gather).
+// Device-side table access: buffer loads through a wave-uniform descriptor
+// over the whole table with 32-bit byte offsets.  (Plain loads from the table
+// pointer become generic-address loads, whose waits also drain LDS traffic.)
+// node() keeps every offset inside the table; the descriptor's record count
+// bounds it as well (out-of-range buffer loads return 0).
 __device__ inline double tabd(const double* tab, int idx) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  typedef __attribute__((address_space(1))) const char gchar;
-  typedef __attribute__((address_space(1))) const double gdbl;
-  return *(const gdbl*)((const gchar*)(uintptr_t)tab + (unsigned)idx * 8u);
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(tab), (short)0, NTAB * NC * 8, 0x00020000);
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (unsigned)idx * 8u, 0, 0));
 #else
   return tab[idx];
 #endif

@@ -411,6 +411,7 @@ struct PhysArgs {
   double dx, dy, dz;
   double dkx;  // 2 pi / lx: kx of column kk is kk * dkx
   double n0, mass, beta, cn, alpha0, cg;
+  double inv_n0, inv_beta, inv_cn;  // reciprocals: the per-point functional multiplies
   double dadn, d2adn2;  // d alpha / d n, d2 alpha / d n2 at n0: start of the per-point Newton iteration
   int ideal, tf, grad;
   const double* fdtab;
@@ -479,23 +480,24 @@ __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs
     const bool live = lok && X::xpos(g, p) >= 0;
     const double n = a.n0 + (double)dn[p];
     if (live && !(n > 0.0)) bad = true;
-    inv[p] = (T)(1.0 / (a.mass * n));
+    // divisions are reciprocal multiplies (1 ulp apart from the quotients)
+    inv[p] = (T)fd::rcp_nr(a.mass * n);
     // mu_loc with its k = 0 constants (ln n0 / beta, alpha0 / beta) dropped:
     // only grad(mu) enters the momentum equation (model.f90:99).
-    double v = a.ideal ? log1p((double)dn[p] / a.n0) / a.beta : 0.0;  // fe_ideal_classical.f90:40
+    double v = a.ideal ? log1p((double)dn[p] * a.inv_n0) * a.inv_beta : 0.0;  // fe_ideal_classical.f90:40
     if (a.tf || a.grad) {
       const double al = live && n > 0.0
-                            ? fd::alpha_from_ip12_dev(a.fdtab, n / a.cn,
+                            ? fd::alpha_from_ip12_dev(a.fdtab, n * a.inv_cn,
                                                       a.alpha0 + (double)dn[p] * (a.dadn + 0.5 * a.d2adn2 * (double)dn[p]))
                             : a.alpha0;
-      if (a.tf) v += (al - a.alpha0) / a.beta;  // fe_kinetic_tfk.f90:80-81
-      if (a.grad) {                             // fe_kinetic_tfk.f90:130-137
+      if (a.tf) v += (al - a.alpha0) * a.inv_beta;  // fe_kinetic_tfk.f90:80-81
+      if (a.grad) {                                 // fe_kinetic_tfk.f90:130-137
         double im, d1, d2;
         fd::im12_d1_d2_dev(a.fdtab, al, im, d1, d2);
-        const double rim = 1.0 / im;
+        const double rim = fd::rcp_nr(im);
         const double xi = d1 * rim * rim;
         const double dxi = (d2 * im - 2.0 * d1 * d1) * (rim * rim * rim);
-        const double dxidn = dxi * rim * (2.0 / a.cn);
+        const double dxidn = dxi * rim * (2.0 * a.inv_cn);
         v -= a.cg * (dxidn * (double)g2[p] + 2.0 * xi * (double)lap[p]);
       }
     }
@@ -601,7 +603,7 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
     if (nd >= 3) X::inv_pair(sm, &a.m[2], nullptr, line, lok, a.Kx, a.dkx, g, lw, m2, nullptr);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) {
-      const double inv = 1.0 / (a.mass * (a.n0 + dn[p]));
+      const double inv = fd::rcp_nr(a.mass * (a.n0 + dn[p]));
       u0[p] = m0[p] * inv; u1[p] = nd >= 2 ? m1[p] * inv : 0.0; u2[p] = m2[p] * inv;
     }
   }
@@ -641,7 +643,7 @@ __device__ __attribute__((always_inline)) inline void umax_body(const PhysArgs* 
 #pragma unroll
   for (int p = 0; p < PTS; ++p) {
     if (!lok || X::xpos(g, p) < 0) continue;
-    const double inv = 1.0 / (a.mass * (a.n0 + dn[p]));
+    const double inv = fd::rcp_nr(a.mass * (a.n0 + dn[p]));
     um = fmax(um, fabs(m0[p] * inv));
     if (nd >= 2) um = fmax(um, fabs(m1[p] * inv));
     if (nd >= 3) um = fmax(um, fabs(m2[p] * inv));
@@ -1301,6 +1303,7 @@ static PhysArgs phys_base(Model& m) {
   a.nd = g.ndim; a.Kx = g.Kx; a.ny = g.ny; a.nz = g.nz;
   a.dx = g.dx; a.dy = g.dy; a.dz = g.dz; a.dkx = 2.0 * fd::PI / g.lx;
   a.n0 = m.n0; a.mass = m.mass; a.beta = m.beta; a.cn = m.cn; a.alpha0 = m.alpha0; a.cg = m.cg;
+  a.inv_n0 = 1.0 / m.n0; a.inv_beta = m.beta > 0.0 ? 1.0 / m.beta : 0.0; a.inv_cn = m.cn > 0.0 ? 1.0 / m.cn : 0.0;
   a.dadn = m.dadn;
   a.d2adn2 = m.d2adn2;
   a.ideal = m.ideal; a.tf = m.tf; a.grad = m.grad;
