@@ -751,18 +751,24 @@ __global__ void k_reduce_lines_b(const RedK* __restrict__ ks) { reduce_lines_bod
 // ------------------------------------------------------------ spectral kernels
 struct SpecArgs {
   int Kx, Ky, Kz, My, Mz, ny, nz, nd;
-  const double *kx, *ky, *kz;
-  size_t nt;   // Kx*Ky*Kz
+  double dkx, dky, dkz;  // 2 pi / L_d (0 for an absent axis): k_d = signed mode index * dk_d
+  size_t nt;   // Kx*Ky*Kz (< 2^31 for every supported grid)
 };
+// Wavenumbers of kept-box entry e (grid.f90:77-88: k = 2 pi wrap(m) / L).
+// 32-bit index arithmetic and no table loads: a 64-bit division by a run-time
+// value costs about a hundred instructions, a generic-address table load a
+// full wait.
 __device__ inline void spec_k(const SpecArgs& a, size_t e, double k[3]) {
-  int ix = (int)(e % a.Kx);
-  int ty = (int)((e / a.Kx) % a.Ky);
-  int tz = (int)(e / ((size_t)a.Kx * a.Ky));
-  int jy = ty <= a.My ? ty : ty - a.Ky + a.ny;
-  int jz = tz <= a.Mz ? tz : tz - a.Kz + a.nz;
-  k[0] = a.kx[ix];
-  k[1] = a.ky[jy];
-  k[2] = a.kz[jz];
+  const unsigned u = (unsigned)e, kxn = (unsigned)a.Kx, kyn = (unsigned)a.Ky;
+  const unsigned q = u / kxn;
+  const int ix = (int)(u - q * kxn);
+  const unsigned tz = q / kyn;
+  const int ty = (int)(q - tz * kyn);
+  const int sy = ty <= a.My ? ty : ty - a.Ky;
+  const int sz = (int)tz <= a.Mz ? (int)tz : (int)tz - a.Kz;
+  k[0] = ix * a.dkx;
+  k[1] = sy * a.dky;
+  k[2] = sz * a.dkz;
 }
 template <class T>
 __device__ inline c64 ld(const void* p, size_t i) {
@@ -826,9 +832,9 @@ __device__ __attribute__((always_inline)) inline void spec1_body(const K1Args& a
   }
 }
 template <class T>
-__global__ void k_spec1(K1Args a) { spec1_body<T>(a); }
+__global__ __launch_bounds__(256) void k_spec1(K1Args a) { spec1_body<T>(a); }
 template <class T>
-__global__ void k_spec1_b(const K1Args* __restrict__ as) { spec1_body<T>(as[blockIdx.z]); }
+__global__ __launch_bounds__(256) void k_spec1_b(const K1Args* __restrict__ as) { spec1_body<T>(as[blockIdx.z]); }
 
 struct K2Args {
   SpecArgs s;
@@ -838,7 +844,7 @@ struct K2Args {
   const void* fh[3];      // F(H_i)
   const void* fu[3];      // F(u) (Newtonian stress)
   const void* fadv[6];    // F(adv_c) (Maxwell)
-  double mass, eta, xi;
+  double mass, inv_mass, eta, xi;
   int stress_newt, evolve, ext_active;
   int stage;              // 0..2
   int rhs_only;
@@ -863,7 +869,7 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
 #pragma unroll
     for (int d = 0; d < ND; ++d) km = km + scal(a.S[(1 + d) * nt + e], k[d]);
     c64 r[10];
-    r[0] = mk<double>(km.y / a.mass, -km.x / a.mass);
+    r[0] = mk<double>(km.y * a.inv_mass, -km.x * a.inv_mass);
     // stress divergence f_i = d_j Pi_ij
     c64 pih[6];
 #pragma unroll
@@ -931,9 +937,9 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
   }
 }
 template <class T, int ND>
-__global__ void k_spec2(K2Args a) { spec2_body<T, ND>(a); }
+__global__ __launch_bounds__(256) void k_spec2(K2Args a) { spec2_body<T, ND>(a); }
 template <class T, int ND>
-__global__ void k_spec2_b(const K2Args* __restrict__ as) { spec2_body<T, ND>(as[blockIdx.z]); }
+__global__ __launch_bounds__(256) void k_spec2_b(const K2Args* __restrict__ as) { spec2_body<T, ND>(as[blockIdx.z]); }
 
 // probe: spec_mode of n/n0 and mom_x (diagnostics.f90:70-77) after a step
 __global__ void k_probe(const Ctl* c, const c64* S, size_t nt, long long idx, double n0, double* out) {
@@ -1096,7 +1102,10 @@ static SpecArgs spec_args(const Model& m) {
   SpecArgs a;
   const Grid& g = m.g;
   a.Kx = g.Kx; a.Ky = g.Ky; a.Kz = g.Kz; a.My = g.My; a.Mz = g.Mz; a.ny = g.ny; a.nz = g.nz;
-  a.nd = g.ndim; a.kx = g.d_kx; a.ky = g.d_ky; a.kz = g.d_kz; a.nt = m.nt;
+  a.nd = g.ndim; a.nt = m.nt;
+  a.dkx = 2.0 * fd::PI / g.lx;
+  a.dky = g.ny > 1 ? 2.0 * fd::PI / g.ly : 0.0;
+  a.dkz = g.nz > 1 ? 2.0 * fd::PI / g.lz : 0.0;
   return a;
 }
 static int nblocks(size_t n) { return (int)std::min<size_t>((n + 255) / 256, 16384); }
@@ -1462,7 +1471,7 @@ static int launch_k2(Model& m, int stage_idx, int rhs_only) {
   for (int i = 0; i < 3; ++i) a.fh[i] = fslot(m, FS_B(OB_H1 + i));
   for (int i = 0; i < 3; ++i) a.fu[i] = fslot(m, FS_A(OA_U1 + i));
   for (int c = 0; c < 6; ++c) a.fadv[c] = fslot(m, FS_B(OB_ADV1 + c));
-  a.mass = m.mass; a.eta = m.eta; a.xi = m.xi;
+  a.mass = m.mass; a.inv_mass = 1.0 / m.mass; a.eta = m.eta; a.xi = m.xi;
   a.stress_newt = m.stress_newt; a.evolve = m.evolve;
   a.ext_active = m.vext_kind != HY_VEXT_NONE;
   a.stage = stage_idx;
