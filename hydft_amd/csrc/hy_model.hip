@@ -222,15 +222,16 @@ __device__ inline cpx<T> xin_load(const XIn& in, size_t line, int Kx, int kk, do
   return v;
 }
 
-// ---- x-line transforms on one 64-lane wave
-// An NX-point line is transformed by a Stockham sequence whose first stage
+// ---- x-line transforms on TPC lanes of a wave
+// An NX-point line is transformed by TPC lanes (64 / TPC lines per wave, see
+// XWave) with a Stockham sequence whose first stage
 // reads its operands from the caller (global loads or the lane's registers)
 // and whose last stage hands its results back in registers; only the middle
 // stages exchange data through two LDS buffers (one barrier per stage).  The
 // inverse transforms use the radix order radix_of(NX, s) and the forward ones
 // the reversed order, so the points a lane holds after an inverse transform
-// are exactly the operands of its first forward stage: lane `lane` owns the
-// XQ<NX>::Q points x = xown<NX>(lane, q).
+// are exactly the operands of its first forward stage: lane `lane` (of the
+// line's TPC lanes) owns the XQ<NX, TPC>::Q points x = xown<NX, TPC>(lane, q).
 template <int L, bool REV>
 __host__ __device__ constexpr int prad(int s) { return REV ? radix_of(L, nstages_of(L) - 1 - s) : radix_of(L, s); }
 template <int L, bool REV>
@@ -244,31 +245,31 @@ __host__ __device__ constexpr int pns(int s) { int p = 1; for (int i = 0; i < s;
 // supported NX.
 __device__ constexpr int swz(int e) { return e ^ (((e >> 4) & 3) * 5); }
 
-template <int NX>
+template <int NX, int TPC = 64>
 struct XQ {
   static constexpr int RL = radix_of(NX, nstages_of(NX) - 1);  // last inverse radix
   static constexpr int NBL = NX / RL;
-  static constexpr int PERL = (NBL + 63) / 64;
+  static constexpr int PERL = (NBL + TPC - 1) / TPC;
   static constexpr int Q = PERL * RL;  // points per lane
 };
 // point x of the lane's q-th value, or -1 (lane idle for that slot)
-template <int NX>
+template <int NX, int TPC = 64>
 __device__ inline int xown(int lane, int q) {
-  const int j = lane + 64 * (q / XQ<NX>::RL);
-  return j < XQ<NX>::NBL ? j + (q % XQ<NX>::RL) * XQ<NX>::NBL : -1;
+  const int j = lane + TPC * (q / XQ<NX, TPC>::RL);
+  return j < XQ<NX, TPC>::NBL ? j + (q % XQ<NX, TPC>::RL) * XQ<NX, TPC>::NBL : -1;
 }
 
-template <class T, int L, bool INV, bool REV, int S, class In, class Out>
+template <class T, int L, bool INV, bool REV, int TPC, int S, class In, class Out>
 __device__ inline void xstage(const cpx<T>* src, cpx<T>* dst, const cpx<T>* tw, int lane, In& in, Out& out) {
   constexpr int NST = nstages_of(L);
   constexpr int R = prad<L, REV>(S);
   constexpr int NS = pns<L, REV>(S);
   constexpr int NB = L / R;
-  constexpr int PER = (NB + 63) / 64;
+  constexpr int PER = (NB + TPC - 1) / TPC;
 #pragma unroll
   for (int b = 0; b < PER; ++b) {
-    const int j = lane + 64 * b;
-    if (NB % 64 == 0 || j < NB) {
+    const int j = lane + TPC * b;
+    if (NB % TPC == 0 || j < NB) {
       cpx<T> v[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -297,23 +298,24 @@ __device__ inline void xstage(const cpx<T>* src, cpx<T>* dst, const cpx<T>* tw, 
 }
 
 // stage S reads buf[(S+1)%2] (S > 0) and writes buf[S%2]
-template <class T, int L, bool INV, bool REV, int S = 0, class In, class Out>
+template <class T, int L, bool INV, bool REV, int TPC, int S = 0, class In, class Out>
 __device__ inline void xfft(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, int lane, In& in, Out& out) {
   if constexpr (S < nstages_of(L)) {
-    xstage<T, L, INV, REV, S>(S % 2 ? b0 : b1, S % 2 ? b1 : b0, tw, lane, in, out);
+    xstage<T, L, INV, REV, TPC, S>(S % 2 ? b0 : b1, S % 2 ? b1 : b0, tw, lane, in, out);
     if constexpr (S + 1 < nstages_of(L)) __syncthreads();
-    xfft<T, L, INV, REV, S + 1>(b0, b1, tw, lane, in, out);
+    xfft<T, L, INV, REV, TPC, S + 1>(b0, b1, tw, lane, in, out);
   }
 }
 
 // Inverse x transform of two spectral x-lines packed as a + i b (a c2r pair:
 // DC and Nyquist imaginary parts dropped); the lane's points of a(x) and
-// b(x) are returned in qa[q], qb[q] (qb may be null).
-template <class T, int NX>
+// b(x) are returned in qa[q], qb[q] (qb may be null).  Lanes of a line
+// beyond the last (lok false) read nothing.
+template <class T, int NX, int TPC = 64>
 __device__ inline void x_inv_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, const XIn* A, const XIn* B,
-                                  size_t line, int Kx, double dkx, int lane, T* qa, T* qb) {
+                                  size_t line, bool lok, int Kx, double dkx, int lane, T* qa, T* qb) {
 #pragma unroll
-  for (int q = 0; q < XQ<NX>::Q; ++q) {  // slots of idle lanes stay defined
+  for (int q = 0; q < XQ<NX, TPC>::Q; ++q) {  // slots of idle lanes stay defined
     qa[q] = 0;
     if (qb) qb[q] = 0;
   }
@@ -322,7 +324,7 @@ __device__ inline void x_inv_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, cons
     const bool hi = 2 * e > NX;
     const int kk = hi ? NX - e : e;
     cpx<T> a = mk<T>(0, 0), b = mk<T>(0, 0);
-    if (kk < Kx) {
+    if (lok && kk < Kx) {
       if (A) a = xin_load<T>(*A, line, Kx, kk, dkx);
       if (B) b = xin_load<T>(*B, line, Kx, kk, dkx);
     }
@@ -331,28 +333,29 @@ __device__ inline void x_inv_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, cons
               : mk<T>(a.x - b.y, a.y + b.x);   // a + i b
   };
   auto out = [&](int b, int r, int, cpx<T> v) {
-    const int q = b * XQ<NX>::RL + r;
+    const int q = b * XQ<NX, TPC>::RL + r;
     qa[q] = v.x;
     if (qb) qb[q] = v.y;
   };
-  xfft<T, NX, true, false>(b0, b1, tw, lane, in, out);
+  xfft<T, NX, true, false, TPC>(b0, b1, tw, lane, in, out);
 }
 
 // Forward x transform of two real lines packed as va + i vb, where
 // val(q) = (va, vb) at the lane's q-th point; stores the first Kx
 // coefficients of each, scaled by 1/NX, to outA / outB (compute type).
-template <class T, int NX, class F>
+// Lanes of a line beyond the last (lok false) store nothing.
+template <class T, int NX, int TPC, class F>
 __device__ inline void x_fwd_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, F&& val, void* outA, void* outB,
-                                  size_t line, int Kx, int lane) {
+                                  size_t line, bool lok, int Kx, int lane) {
   constexpr int NST = nstages_of(NX);
   cpx<T>* res = (NST - 1) % 2 ? b1 : b0;  // where the last stage stores
   __syncthreads();
-  auto in = [&](int b, int r, int) -> cpx<T> { return val(b * XQ<NX>::RL + r); };
+  auto in = [&](int b, int r, int) -> cpx<T> { return val(b * XQ<NX, TPC>::RL + r); };
   auto out = [&](int, int, int e, cpx<T> v) { res[swz(e)] = v; };
-  xfft<T, NX, false, true>(b0, b1, tw, lane, in, out);
+  xfft<T, NX, false, true, TPC>(b0, b1, tw, lane, in, out);
   __syncthreads();
   const T sc = (T)(0.5 / NX);
-  for (int k = lane; k < Kx; k += 64) {
+  for (int k = lane; lok && k < Kx; k += TPC) {
     cpx<T> Z = res[swz(k)], Zc = conjc(res[swz((NX - k) % NX)]);
     gstc<T>(outA, line * Kx + k, scal(Z + Zc, sc));
     if (outB) {
@@ -362,39 +365,46 @@ __device__ inline void x_fwd_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, F&& 
   }
 }
 
-__device__ inline double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+// ---- x-line layout of the x-line kernels: LPW lines per 64-lane wave, TPC =
+// 64 / LPW lanes per line, with the Stockham transforms above.  TPC is the
+// smallest power of two that keeps a lane at no more than 4 points of its
+// line (the 256-point case): every lane then works in every stage, where one
+// line per wave would leave 3 of 4 lanes idle at NX = 64.  The kernels take
+// the layout as a policy: the lane's line (line_in_wave) and position g in
+// it, its points x(g, q) for q < PTS, the pair transforms and per-line
+// reductions.
+__host__ __device__ constexpr int x_tpc(int NX) {
+  int t = 1;
+  while (t < 64 && ((NX / radix_of(NX, nstages_of(NX) - 1)) + t - 1) / t * radix_of(NX, nstages_of(NX) - 1) > 4) t *= 2;
+  return t;
 }
-__device__ inline double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
-
-// ---- x-line layout of the x-line kernels: one line per 64-lane wave, the
-// Stockham transforms above.  The kernels take the layout as a policy: the
-// lane's line (line_in_wave) and position g in it, its points x(g, q) for
-// q < PTS, the pair transforms and per-line reductions.
 template <class T, int NX>
 struct XWave {
-  static constexpr int LPW = 1, TPC = 64, PTS = XQ<NX>::Q;
+  static constexpr int TPC = x_tpc(NX), LPW = 64 / TPC, PTS = XQ<NX, TPC>::Q;
   struct Smem {
-    cpx<T> b0[NX], b1[NX], tw[NX];
+    cpx<T> b0[LPW * NX], b1[LPW * NX], tw[NX];
   };
   __device__ static void init(Smem& s, const cpx<T>* twg, int lane) {
     for (int j = lane; j < NX; j += 64) s.tw[swz(j)] = twg[j];
   }
-  __device__ static int xpos(int g, int q) { return xown<NX>(g, q); }
-  __device__ static void inv_pair(Smem& s, const XIn* A, const XIn* B, size_t line, bool, int Kx,
-                                  double dkx, int g, int, T* qa, T* qb) {
-    x_inv_pair<T, NX>(s.b0, s.b1, s.tw, A, B, line, Kx, dkx, g, qa, qb);
+  __device__ static int xpos(int g, int q) { return xown<NX, TPC>(g, q); }
+  __device__ static void inv_pair(Smem& s, const XIn* A, const XIn* B, size_t line, bool lok, int Kx,
+                                  double dkx, int g, int lw, T* qa, T* qb) {
+    x_inv_pair<T, NX, TPC>(s.b0 + lw * NX, s.b1 + lw * NX, s.tw, A, B, line, lok, Kx, dkx, g, qa, qb);
   }
   template <class F>
-  __device__ static void fwd_pair(Smem& s, F&& val, void* oA, void* oB, size_t line, bool, int Kx, int g, int) {
-    x_fwd_pair<T, NX>(s.b0, s.b1, s.tw, val, oA, oB, line, Kx, g);
+  __device__ static void fwd_pair(Smem& s, F&& val, void* oA, void* oB, size_t line, bool lok, int Kx, int g, int lw) {
+    x_fwd_pair<T, NX, TPC>(s.b0 + lw * NX, s.b1 + lw * NX, s.tw, val, oA, oB, line, lok, Kx, g);
   }
-  __device__ static double line_sum(double v) { return wave_sum(v); }
-  __device__ static double line_max(double v) { return wave_max(v); }
+  // sums / maxima over the TPC lanes of a line (aligned groups of the wave)
+  __device__ static double line_sum(double v) {
+    for (int o = TPC / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+  }
+  __device__ static double line_max(double v) {
+    for (int o = TPC / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+  }
 };
 
 // ---------------------------------------------------------- physics args
