@@ -181,6 +181,11 @@ def _solve_direct(pot, n0, nr, rmax, tol, mix, maxiter, gamma_s_init=None):
             gs = gamma_s_init.copy()
     if unstable:
         return None, gs
+    return _finish(pot, n0, fb, bvs, bvl_r, bvl_k, gs, converged, min(it_total, maxiter), resid), gs
+
+
+def _finish(pot, n0, fb, bvs, bvl_r, bvl_k, gs, converged, niter, resid, expcap=40.0):
+    """g, c(r), c(k), h(k), S(k) and the excess energy from the final gamma_s."""
     g = np.exp(np.minimum(-bvs + gs, expcap))
     cs = g - 1.0 - gs
     ck = fb.r2k(cs) - bvl_k
@@ -188,8 +193,7 @@ def _solve_direct(pot, n0, nr, rmax, tol, mix, maxiter, gamma_s_init=None):
     sk = 1.0 + n0 * hk
     cr = cs - bvl_r
     u_ex = 2.0 * PI * n0 * (fb.dr * float(np.sum(fb.r ** 2 * (g - 1.0) * pot.beta_v(fb.r))))
-    res = HncResult(fb.r, fb.k, g, cr, ck, hk, sk, u_ex, converged, min(it_total, maxiter), resid)
-    return res, gs
+    return HncResult(fb.r, fb.k, g, cr, ck, hk, sk, u_ex, converged, niter, resid)
 
 
 def hnc_solve(pot: PairPotential, n0, nr=8192, rmax=80.0, tol=1.0e-9, mix=0.2, maxiter=20000):
@@ -211,6 +215,42 @@ def hnc_solve(pot: PairPotential, n0, nr=8192, rmax=80.0, tol=1.0e-9, mix=0.2, m
             break
         g = min(gtarget, g * 1.3)
     return res
+
+
+def hnc_solve_many_device(pots, n0s, nr=8192, rmax=80.0, tol=1.0e-9, mix=0.2, maxiter=20000):
+    """hnc_solve for many state points on one radial grid: the direct solves
+    run together on the GPU (hy_hnc_iterate: the same Picard + Ng iteration,
+    restarts and rules as _solve_direct, with the DST-I as a matrix product
+    on the device); a state point whose direct solve is unstable in every
+    attempt continues in Gamma on the host (hnc_solve).  Returns HncResults
+    in input order."""
+    import ctypes as C
+    from .. import abi
+    fb = FourierBessel(nr, rmax)
+    m = len(pots)
+    bvs = np.empty((m, nr)); bvl_r = np.empty((m, nr)); bvl_k = np.empty((m, nr)); gs0 = np.empty((m, nr))
+    for i, (pot, n0) in enumerate(zip(pots, n0s)):
+        bvs[i] = pot.beta_v_short(fb.r)
+        bvl_r[i] = pot.beta_v_long_r(fb.r)
+        bvl_k[i] = pot.beta_v_long_k(fb.k)
+        ck = -pot.beta_v_k(fb.k)
+        gs0[i] = fb.k2r(n0 * ck * ck / (1.0 - n0 * ck) - bvl_k[i])
+    n0a = np.ascontiguousarray(n0s, dtype=float)
+    gs = np.empty((m, nr))
+    status = np.empty(m, dtype=np.int32)
+    niter = np.empty(m, dtype=np.int32)
+    resid = np.empty(m)
+    abi.call("hy_hnc_iterate", m, nr, fb.dr, fb.dk, n0a.ctypes.data, bvs.ctypes.data, bvl_k.ctypes.data,
+             gs0.ctypes.data, float(tol), float(mix), int(maxiter), gs.ctypes.data,
+             status.ctypes.data_as(C.POINTER(C.c_int)), niter.ctypes.data_as(C.POINTER(C.c_int)), resid.ctypes.data)
+    out = []
+    for i, (pot, n0) in enumerate(zip(pots, n0s)):
+        if status[i] < 0:
+            out.append(hnc_solve(pot, n0, nr, rmax, tol, mix, maxiter))
+        else:
+            out.append(_finish(pot, n0, fb, bvs[i], bvl_r[i], bvl_k[i], gs[i], bool(status[i] == 1),
+                               int(niter[i]), float(resid[i])))
+    return out
 
 
 @dataclass
@@ -343,17 +383,20 @@ class StructureCache:
         self._store(key, st)
         return st
 
-    def solve_many(self, requests, processes=None):
+    def solve_many(self, requests, processes=None, device=False):
         """requests: (pot, n0, nr, rmax, tol, mix, maxiter, with_derivative)
-        tuples.  Solves the keys not yet cached in a process pool and returns
-        the Structures in request order."""
+        tuples.  Solves the keys not yet cached, in a process pool or, with
+        `device`, together on the GPU (hnc_solve_many_device), and returns the
+        Structures in request order."""
         import concurrent.futures as cf
         keys = [_structure_key(*r) for r in requests]
         todo = {}
         for key, r in zip(keys, requests):
             if key not in self._mem and self._load(key) is None and key not in todo:
                 todo[key] = r
-        if todo:
+        if todo and device:
+            self._solve_device(todo)
+        elif todo:
             workers = processes or min(len(todo), 8)
             with cf.ProcessPoolExecutor(max_workers=workers) as ex:
                 futs = {key: ex.submit(structure_from_hnc, *r) for key, r in todo.items()}
@@ -361,3 +404,35 @@ class StructureCache:
                     self._store(key, f.result())
                     self.misses += 1
         return [self.get(*r) for r in requests]
+
+    def _solve_device(self, todo):
+        """structure_from_hnc for every key, grouped by solver settings; the
+        derivative runs (Gamma +- 2 %) join the same device batch."""
+        groups = {}
+        for key, r in todo.items():
+            groups.setdefault(tuple(r[2:7]), []).append((key, r))
+        for (nr, rmax, tol, mix, maxiter), items in groups.items():
+            pots, n0s, slots = [], [], []
+            for key, (pot, n0, *_rest, with_derivative) in items:
+                idx = [len(pots)]
+                pots.append(pot); n0s.append(n0)
+                if with_derivative:
+                    dg = 0.02 * pot.gamma
+                    for gg in (pot.gamma + dg, pot.gamma - dg):
+                        idx.append(len(pots))
+                        pots.append(PairPotential(pot.name, gg, pot.kappa, pot.ksplit, pot.lam, pot.pauli))
+                        n0s.append(n0)
+                slots.append((key, pot, n0, with_derivative, idx))
+            res = hnc_solve_many_device(pots, n0s, nr, rmax, tol, mix, maxiter)
+            for key, pot, n0, with_derivative, idx in slots:
+                r0 = res[idx[0]]
+                if not r0.converged:
+                    raise HncError("structure: HNC did not converge")
+                st = Structure(k=r0.k.copy(), c=r0.ck.copy(), n0=n0, u_ex=r0.u_ex, has_energy=True)
+                if with_derivative:
+                    rp1, rm1 = res[idx[1]], res[idx[2]]
+                    if not (rp1.converged and rm1.converged):
+                        raise HncError("structure: HNC derivative runs did not converge")
+                    st.du_dgamma = (rp1.u_ex - rm1.u_ex) / (2.0 * 0.02 * pot.gamma)
+                self._store(key, st)
+                self.misses += 1

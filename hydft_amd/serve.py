@@ -29,6 +29,10 @@ library refuses as a batch runs as separate captured runs.  Batching changes
 how the kernels are launched, not what they compute: each run's result is
 bitwise the result it has alone.
 
+With `hnc_device=True` the c(k) of arrived requests that are not yet in the
+structure cache are solved together on the GPU (hnc.hnc_solve_many_device)
+before they are admitted, instead of one host solve per model.
+
 `one_at_a_time` is the reference's way of running the same requests: one
 model at a time, steps taken by hy_model_advance calls.
 """
@@ -57,7 +61,7 @@ class Request:
     kind: str = "chi"            # "chi" | "dispersion"
     mode: tuple = (1, 0, 0)      # probed (and driven / excited) mode
     arrival: float = 0.0         # seconds after the start of the stream
-    t_fit: float = -1.0          # analysis window start; < 0: the second half of the run
+    t_fit: float = -1.0          # analysis window start; < 0: the second half of the samples
 
 
 @dataclass
@@ -106,7 +110,11 @@ def make_model(req: Request, precision: str, cache=None):
 def analyse(req: Request, rows: np.ndarray) -> dict:
     t = rows[:, 0]
     z = rows[:, 1] + 1j * rows[:, 2]
-    t0 = req.t_fit if req.t_fit >= 0.0 else 0.5 * t[-1]
+    # default window: the second half of the samples, by index (a start time of
+    # t_end / 2 would fall exactly on a sample at a constant dt, and rounding
+    # would then decide whether that sample is in)
+    n = len(t)
+    t0 = req.t_fit if req.t_fit >= 0.0 else (0.5 * (t[n // 2 - 1] + t[n // 2]) if n >= 2 else t[0])
     if req.kind == "chi":
         a, scatter = fit_driven_response(t, z, req.p.vext_omega, t0)
         return {"chi": 2.0 * a / req.p.vext_amplitude, "scatter": scatter}
@@ -146,7 +154,8 @@ def batch_key(req: Request) -> tuple:
 
 class Server:
     def __init__(self, hbm_budget: float = 200e9, max_slots: int = 64, chunk: int = DT_EVERY,
-                 precision: str = "mixed", cache=None, batch: bool = False, max_batch: int = 256):
+                 precision: str = "mixed", cache=None, batch: bool = False, max_batch: int = 256,
+                 hnc_device: bool = False):
         self.hbm_budget = float(hbm_budget)
         self.max_slots = int(max_slots)
         self.chunk = int(chunk)
@@ -154,6 +163,7 @@ class Server:
         self.cache = cache if cache is not None else _hnc.StructureCache()
         self.batch = bool(batch)
         self.max_batch = int(max_batch)
+        self.hnc_device = bool(hnc_device)
         self.peak_active = 0      # runs in flight
         self.peak_bytes = 0
         self.batches_launched = 0  # slots: batches and single runs
@@ -223,6 +233,8 @@ class Server:
             progressed = False
             if not active and self._retired:
                 used -= self._release()
+            if self.hnc_device and pending and (not realtime or pending[0].arrival <= now):
+                self._prefetch([r for r in pending if not realtime or r.arrival <= now])
             while pending and len(active) < self.max_slots and (not realtime or pending[0].arrival <= now):
                 need = estimate_bytes(pending[0].p, self.precision)
                 if active and used + need > self.hbm_budget and self._retired:
@@ -253,6 +265,13 @@ class Server:
                 time.sleep(0.0002)
         used -= self._release()
         return results
+
+    def _prefetch(self, reqs):
+        """c(k) of the requests that need an HNC solve, in one device batch."""
+        from .model import hnc_request
+        want = [hnc_request(r.p) for r in reqs if r.p.structure == "hnc"]
+        if want:
+            self.cache.solve_many(want, device=True)
 
     def _release(self) -> int:
         freed = 0

@@ -218,6 +218,25 @@ int hy_batch_sync(hy_batch b);
 int hy_batch_destroy(hy_batch b);
 int hy_model_probe_ring(hy_model m, int ring, double* h_probe);
 
+/* ---- batched OZ-HNC direct solve (oz_hnc.f90:111-140 direct branch, the
+   iteration of oz_hnc.f90:142-296) --------------------------------------
+   nsolve independent solves on one radial grid r_j = (j+1) dr, k_m = (m+1) dk
+   (dk = pi / ((nr+1) dr)), iterated on the GPU: per solve, n0, beta v_short(r)
+   (h_bvs, nsolve x nr), beta v_long(k) (h_bvlk) and the initial gamma_s
+   (h_gs0, the mean-field guess or a continuation start).  Picard + Ng
+   iteration with the reference's rules (Ng after 5 iterations with three
+   history entries, cooldown of 10 after a residual jump, Ng step only when
+   max(gamma_s - beta v_s) < 40), convergence when max|gamma_new - gamma_s|
+   < tol, and up to 4 attempts with halved mixing from h_gs0 on instability.
+   Returns the final gamma_s (nsolve x nr) and, per solve, status 1
+   (converged), 0 (maxiter reached) or -1 (unstable in every attempt: the
+   caller continues in Gamma on the host), the iterations of the last
+   attempt and the last residual.  No counterpart as one call in the
+   reference (it solves one state point at a time on the host). */
+int hy_hnc_iterate(int nsolve, int nr, double dr, double dk, const double* h_n0, const double* h_bvs,
+                   const double* h_bvlk, const double* h_gs0, double tol, double mix, int maxiter, double* h_gs,
+                   int* h_status, int* h_niter, double* h_resid);
+
 #ifdef __cplusplus
 }
 #endif
