@@ -158,6 +158,14 @@ module hydft_hip_model
       type(c_ptr), value :: m, probe
       integer(c_int), value :: ring
     end function
+    ! batched OZ-HNC direct solve on the GPU (oz_hnc.f90:111-296 for many state points)
+    integer(c_int) function hy_hnc_iterate(nsolve, nr, dr, dk, n0, bvs, bvlk, gs0, tol, mix, maxiter, gs, status, &
+        niter, resid) bind(C, name='hy_hnc_iterate')
+      import :: c_int, c_ptr, c_double
+      integer(c_int), value :: nsolve, nr, maxiter
+      real(c_double), value :: dr, dk, tol, mix
+      type(c_ptr), value :: n0, bvs, bvlk, gs0, gs, status, niter, resid
+    end function
   end interface
 
 end module hydft_hip_model
