@@ -41,6 +41,10 @@ CASES = {
                              eta=0.3, xi=0.1, dt=0.01), 2),
     "ion3d_maxwell_y48": (dict(nx=16, ny=48, nz=8, lx=12.0, ly=30.0, lz=6.0, gamma=20.0, kappa=1.0, eta=0.2,
                                closure="maxwell", tau_model="constant", tau=0.5, dt=0.005), 2),
+    # 128-point x lines (two lines per wave) with the Maxwell stress; the
+    # 1-D cases above leave all but one line of a wave empty
+    "ion3d_maxwell_nx128": (dict(nx=128, ny=12, nz=4, lx=40.0, ly=9.0, lz=6.0, gamma=50.0, kappa=2.0, eta=0.2,
+                                 closure="maxwell", tau_model="constant", tau=0.5, dt=0.005), 2),
 }
 
 # FP32 transforms with the FP64 RK3 state: density error against the FP64
