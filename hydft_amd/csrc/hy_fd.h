@@ -258,18 +258,52 @@ __device__ inline double rcp_nr(double d) {
 #endif
 }
 
+// The I_{1/2} interpolant at a inside the table and its first two
+// alpha-derivatives: the exact derivatives of the cubic piece of node i
+// (the piece the Newton iteration solves on).
+__device__ inline void ip12_d2_dev(const double* tab, double a, int i, double& ip, double& dp, double& d2p) {
+  const int o = i * NC;
+  const double t = (a - (AMIN + i * H)) * INVH;
+  const double t2 = t * t, t3 = t2 * t;
+  const double h00 = 2 * t3 - 3 * t2 + 1, h10 = (t3 - 2 * t2 + t) * H, h01 = -2 * t3 + 3 * t2,
+               h11 = (t3 - t2) * H;
+  const double g00 = 6 * t2 - 6 * t, g10 = (3 * t2 - 4 * t + 1) * H, g01 = -6 * t2 + 6 * t, g11 = (3 * t2 - 2 * t) * H;
+  const double k00 = 12 * t - 6, k10 = (6 * t - 4) * H, k01 = -12 * t + 6, k11 = (6 * t - 2) * H;
+  const double q0 = 0.5 * tabd(tab, o), p0 = tabd(tab, o + 1);
+  const double q1 = 0.5 * tabd(tab, o + NC), p1 = tabd(tab, o + NC + 1);
+  ip = h00 * p0 + h10 * q0 + h01 * p1 + h11 * q1;
+  dp = (g00 * p0 + g10 * q0 + g01 * p1 + g11 * q1) * INVH;
+  d2p = (k00 * p0 + k10 * q0 + k01 * p1 + k11 * q1) * (INVH * INVH);
+}
+
 // alpha_from_ip12_from with device gathers: the same iteration and safeguard;
-// the step's division is the hardware reciprocal refined once.  Besides the
-// reference's rule (stop before a step below 1e-14 max(1, |alpha|)), a Newton
-// step below 2e-8 max(1, |alpha|) that needed no safeguard is taken and ends
-// the iteration: by quadratic convergence the error after it is at most
-// |I''/(2 I')| step^2 <= 2e-16 max(1, |alpha|)^2 (|I''/(2 I')| <= 1/2 on the
-// whole range), below the 1e-14 of the reference's own stopping point.  From
-// the field kernels' second-order start this is two table evaluations per
-// point instead of three.
+// the step's division is the hardware reciprocal refined once.  It first
+// takes one Halley step on the cubic piece that holds the start, with the
+// piece's own first and second derivatives; Halley's error on a cubic is
+// at most about (p''/(2p'))^2 + |p'''/(6p')| <= 0.5 times step^3, so a step
+// below 1e-5 max(1, |alpha|) that stays in the piece ends the solve (error
+// <= 5e-16 max(1, |alpha|)^3, below the 1e-14 of the reference's stopping
+// point).  Otherwise the reference's Newton iteration continues from the
+// better of the two points; besides its rule (stop before a step below
+// 1e-14 max(1, |alpha|)), a Newton step below 2e-8 max(1, |alpha|) that
+// needed no safeguard ends it: the error after it is at most
+// |I''/(2 I')| step^2 <= 2e-16 max(1, |alpha|)^2 (|I''/(2 I')| <= 1/2).
+// From the field kernels' second-order start this is one table evaluation
+// per point at the density excursions of a run (|dn/n0| of a few 1e-3).
 __device__ inline double alpha_from_ip12_dev(const double* tab, double y, double a_start) {
   const double huge = 1.7976931348623157e308;
   double alpha = a_start;
+  if (!(alpha > AMAX || alpha < AMIN)) {
+    const int i = node(alpha);
+    double ip, dp, d2p;
+    ip12_d2_dev(tab, alpha, i, ip, dp, d2p);
+    const double f = ip - y;
+    const double step = 2.0 * f * dp * rcp_nr(2.0 * dp * dp - f * d2p);
+    if (fabs(step) < 1.0e-14 * fmax(1.0, fabs(alpha))) return alpha;
+    const double an = alpha - step;
+    if (!(an > AMAX || an < AMIN) && node(an) == i && fabs(step) < 1.0e-5 * fmax(1.0, fabs(an))) return an;
+    if (fabs(step) < 0.5) alpha = an;  // a sound start for the safeguarded iteration
+  }
   double lo = -huge, hi = huge;
   for (int it = 0; it < 100; ++it) {
     double ip, im;
