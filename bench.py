@@ -260,6 +260,56 @@ def serve_sample(nreq, steps, grids=(32, 48, 64)):
             "cold_ck_chi_max_rel_diff": chi_diff}
 
 
+def maxwell_sample(n=128, steps=1000):
+    """The 128^3 Maxwell configuration: Yukawa ions, Gamma = 50, kappa = 2,
+    Maxwell closure with Pi as state (Stanton-Murillo eta, Ichimaru tau from
+    the HNC excess energy), dt = 0.005, 1000 steps, FP64 and FP32-mixed from
+    the same state.  Initial dn: a seeded Gaussian random field with spectral
+    shape exp(-q^2/2) and random phases, zero mean, scaled to an rms of
+    1e-2 n0 (an amplitude of 1e-2 n0 on every mode would drive the density
+    negative at this box size); u = 0, Pi = 0.  Returns steps/s of both and
+    the mixed density error against FP64 after the run."""
+    from hydft_amd import abi
+    from hydft_amd.host.params import Params
+    from hydft_amd.model import Model, build_species
+    p = Params()
+    p.kind, p.gamma, p.kappa = "ion", 50.0, 2.0
+    p.nx = p.ny = p.nz = n
+    p.lx = p.ly = p.lz = 2.0 * math.pi / 0.2
+    p.closure, p.transport, p.tau_model = "maxwell", "stanton_murillo", "ichimaru"
+    p.dt = 0.005
+    p.finalize()
+    sp = build_species(p)
+    rng = np.random.default_rng(2024)
+    w = rng.standard_normal((n, n, n))
+    kx = 2.0 * math.pi * np.fft.rfftfreq(n, p.lx / n)
+    ky = 2.0 * math.pi * np.fft.fftfreq(n, p.ly / n)
+    kz = 2.0 * math.pi * np.fft.fftfreq(n, p.lz / n)
+    q2 = kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2
+    dn = np.fft.irfftn(np.fft.rfftn(w) * np.exp(-0.5 * q2), s=(n, n, n))
+    dn -= dn.mean()
+    n0 = sp.units.n0
+    dn *= 1e-2 * n0 / np.sqrt(np.mean(dn * dn))
+    out = {"grid": n, "steps": steps, "dt": p.dt}
+    fields = {}
+    for prec in ("fp64", "mixed"):
+        m = Model(p, species=sp, precision=prec)
+        m.set_state(n0 + dn)
+        m.advance(5)
+        abi.call("hy_device_sync")
+        t0 = time.perf_counter()
+        m.advance(steps - 5)
+        abi.call("hy_device_sync")
+        out[f"{prec}_steps_per_s"] = (steps - 5) / (time.perf_counter() - t0)
+        fields[prec] = m.get_state()[0]
+        m.close()
+    a, b = fields["fp64"], fields["mixed"]
+    out["mixed_over_fp64"] = out["mixed_steps_per_s"] / out["fp64_steps_per_s"]
+    out["mixed_density_rel_l2_vs_fp64"] = float(np.linalg.norm(b - a) / np.linalg.norm(a))
+    out["mixed_dn_rel_l2_vs_fp64"] = float(np.linalg.norm(b - a) / np.linalg.norm(a - n0))
+    return out
+
+
 def batch_only(spec, steps):
     """Step one batch of same-shape driven-mode runs (spec "B,N,closure") and
     print model-steps/s: the batched kernels alone, for profiling."""
@@ -303,6 +353,7 @@ def main():
     ap.add_argument("--serve", type=int, default=96, metavar="NREQ",
                     help="batched/served chi(k, w) sample size (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=400)
+    ap.add_argument("--no-maxwell", action="store_true", help="skip the 128^3 Maxwell FP64/mixed sample")
     ap.add_argument("--batch-only", default="", metavar="B,N,CLOSURE",
                     help="only step one batch of B driven-mode runs on N^3 (profiling)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
@@ -383,6 +434,8 @@ def main():
         m64.close()
     if args.serve > 0:
         extra["serve"] = serve_sample(args.serve, args.serve_steps)
+    if not args.no_maxwell:
+        extra["maxwell_128"] = maxwell_sample()
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(p, st, n, mom)
