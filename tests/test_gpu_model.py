@@ -45,6 +45,11 @@ CASES = {
     # 1-D cases above leave all but one line of a wave empty
     "ion3d_maxwell_nx128": (dict(nx=128, ny=12, nz=4, lx=40.0, ly=9.0, lz=6.0, gamma=50.0, kappa=2.0, eta=0.2,
                                  closure="maxwell", tau_model="constant", tau=0.5, dt=0.005), 2),
+    # 256-point x lines (one line per wave, the bench's length) on thin 3-D grids
+    "ion3d_nx256_visc": (dict(nx=256, ny=8, nz=4, lx=60.0, ly=6.0, lz=4.0, gamma=5.0, kappa=1.0,
+                              eta=0.3, xi=0.1, dt=0.01), 2),
+    "elec3d_nx256": (dict(kind="electron", gamma=1.0, rs=1.86, nx=256, ny=8, nz=4, lx=60.0, ly=8.0, lz=6.0,
+                          eta=0.1, dt=0.01), 2),
 }
 
 # FP32 transforms with the FP64 RK3 state: density error against the FP64
