@@ -249,6 +249,7 @@ def serve_sample(nreq, steps, grids=(32, 48, 64)):
             "batched_over_one_at_a_time": s_b["runs_per_s"] / s_one["runs_per_s"],
             "batched_launch_groups": srv.batches_launched,
             "batched_latency_p50_s": s_b["latency_p50_s"], "batched_latency_p99_s": s_b["latency_p99_s"],
+
             "served_arrival_rate_per_s": rate, "served_runs_per_s": s_s["runs_per_s"],
             "served_launch_groups": srv_rt.batches_launched,
             "served_latency_p50_s": s_s["latency_p50_s"], "served_latency_p99_s": s_s["latency_p99_s"],

@@ -107,7 +107,27 @@ def make_model(req: Request, precision: str, cache=None):
     return m
 
 
-def analyse(req: Request, rows: np.ndarray) -> dict:
+def linear_for_model(m):
+    """Closed-form linear response of a model's species (linear_response.f90:
+    59-147): the functional kernel K(k) of the model, eta_l = 4/3 eta + xi per
+    particle, the Maxwell time."""
+    from .host.linear import Linear
+    sp, un = m.sp, m.sp.units
+    eta_l = (4.0 * sp.eta_code / 3.0 + sp.xi_code) / un.n0
+    return Linear(un.n0, lambda k: np.array([m.kernel(float(np.atleast_1d(k)[0]))]), m.p.mass, eta_l, sp.tau,
+                  un.beta, un.hbar, un.quantum)
+
+
+def _mode_k(req: Request) -> float:
+    p = req.p
+    kx, ky, kz = (TWOPI * req.mode[0] / p.lx, TWOPI * req.mode[1] / p.ly, TWOPI * req.mode[2] / p.lz)
+    return math.sqrt(kx * kx + ky * ky + kz * kz)
+
+
+def analyse(req: Request, rows: np.ndarray, lin=None) -> dict:
+    """chi (kind "chi") or omega, gamma (kind "dispersion") from the probe
+    series; with `lin` (linear_for_model), the closed form beside it as the
+    per-request check."""
     t = rows[:, 0]
     z = rows[:, 1] + 1j * rows[:, 2]
     # default window: the second half of the samples, by index (a start time of
@@ -117,9 +137,18 @@ def analyse(req: Request, rows: np.ndarray) -> dict:
     t0 = req.t_fit if req.t_fit >= 0.0 else (0.5 * (t[n // 2 - 1] + t[n // 2]) if n >= 2 else t[0])
     if req.kind == "chi":
         a, scatter = fit_driven_response(t, z, req.p.vext_omega, t0)
-        return {"chi": 2.0 * a / req.p.vext_amplitude, "scatter": scatter}
+        out = {"chi": 2.0 * a / req.p.vext_amplitude, "scatter": scatter}
+        if lin is not None:
+            cl = lin.chi(_mode_k(req), req.p.vext_omega)
+            out["chi_linear"] = cl
+            out["chi_rel_dev"] = float(abs(out["chi"] - cl) / abs(cl))
+        return out
     om, ga, nz = fit_damped_mode(t, z.real, t0)
-    return {"omega": om, "gamma": ga, "nzeros": nz}
+    out = {"omega": om, "gamma": ga, "nzeros": nz}
+    if lin is not None:
+        w, _ = lin.dispersion(_mode_k(req))
+        out["omega_linear"], out["gamma_linear"] = w.real, w.imag
+    return out
 
 
 @dataclass
@@ -257,7 +286,7 @@ class Server:
                     rows = s.rows(i)[: req.nsteps]
                     done = time.perf_counter() - t0
                     results.append(Result(req.rid, req.kind, req.arrival, s.start, done, s.steps, rows,
-                                          analyse(req, rows)))
+                                          analyse(req, rows, linear_for_model(m))))
                 active.remove(s)
                 self._retired.append(s)
                 progressed = True
@@ -292,15 +321,21 @@ def one_at_a_time(requests, precision: str = "mixed", cache=None):
         m = make_model(req, precision, cache)
         taken, rows = m.advance(req.nsteps, dt_every=DT_EVERY if req.p.dt <= 0.0 else 0, probe=req.mode)
         done = time.perf_counter() - t0
-        results.append(Result(req.rid, req.kind, req.arrival, start, done, taken, rows, analyse(req, rows)))
+        results.append(Result(req.rid, req.kind, req.arrival, start, done, taken, rows,
+                              analyse(req, rows, linear_for_model(m))))
         m.close()
     return results
 
 
 def summary(results, wall: float) -> dict:
     lat = np.array([r.latency for r in results])
-    return {"runs": len(results), "wall_s": wall, "runs_per_s": len(results) / wall,
-            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99))}
+    out = {"runs": len(results), "wall_s": wall, "runs_per_s": len(results) / wall,
+           "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99))}
+    dev = [r.value["chi_rel_dev"] for r in results if "chi_rel_dev" in r.value]
+    if dev:
+        out["chi_vs_linear_median"] = float(np.median(dev))
+        out["chi_vs_linear_max"] = float(np.max(dev))
+    return out
 
 
 def chi_requests(n: int, seed: int = 7, grid=(32,), steps=400, arrival_rate=None, kind="ion") -> list:

@@ -83,3 +83,34 @@ def test_served_matches_one_at_a_time(gpu):
     b = {r.rid: r for r in serve.Server(precision="fp64", cache=cache).serve(reqs, realtime=False)}
     for rid in a:
         assert np.array_equal(a[rid].probe, b[rid].probe), rid
+
+
+@pytest.mark.gpu
+def test_dispersion_request_matches_linear_theory(gpu):
+    """A served single-mode dispersion request, in the manner of
+    test_mode_newtonian.f90 (ions, Gamma = 2, kappa = 0.1, nx = 32, lx = 20,
+    mode 2, amplitude 1e-4, cfl 0.1, t = 100, fit from t = 5): frequency within
+    1 % and damping within 3 % of the closed form the result carries."""
+    from hydft_amd.host.params import Params
+    p = Params()
+    p.kind, p.gamma, p.kappa = "ion", 2.0, 0.1
+    p.nx, p.lx = 32, 20.0
+    p.transport, p.eta, p.closure = "constant", 0.1, "newtonian"
+    p.hnc_nr, p.hnc_rmax = 4096, 60.0
+    p.cfl, p.dt = 0.1, 0.0
+    p.ic_amplitude = 1.0e-4
+    p.finalize()
+    req = serve.Request(0, p, nsteps=1200, kind="dispersion", mode=(2, 0, 0), t_fit=5.0)
+    res = serve.Server(precision="fp64", batch=True).serve([req], realtime=False)[0]
+    v = res.value
+    assert res.probe[-1, 0] > 90.0
+    assert abs(v["omega"] / v["omega_linear"] - 1.0) < 0.01, v
+    assert abs(v["gamma"] / v["gamma_linear"] - 1.0) < 0.03, v
+
+
+@pytest.mark.gpu
+def test_served_chi_carries_linear_theory(gpu):
+    reqs = [_req(i, N=16, steps=40) for i in range(2)]
+    out = serve.Server(precision="mixed", batch=True).serve(reqs, realtime=False)
+    for r in out:
+        assert "chi_linear" in r.value and np.isfinite(r.value["chi_rel_dev"])
