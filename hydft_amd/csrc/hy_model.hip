@@ -2424,6 +2424,9 @@ int hy_batch_create(const hy_model* ms, int n, int nsteps, int dt_every, const i
   int maxz = 0;
   HY_HIP(hipDeviceGetAttribute(&maxz, hipDeviceAttributeMaxGridDimZ, 0));
   if (n > maxz) return set_error(HY_ERR_INVALID, "hy_batch_create: too many models for one launch");
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j)
+      if (ms[i] == ms[j]) return set_error(HY_ERR_INVALID, "hy_batch_create: a model appears twice");
   std::vector<Recorder> recs(n);
   for (int i = 0; i < n; ++i) {
     if (!ms[i]) return set_error(HY_ERR_INVALID, "hy_batch_create: null model");

@@ -203,7 +203,8 @@ int hy_model_device_bytes(hy_model m, unsigned long long* bytes);
    hy_batch_create records nsteps steps of each model as hy_model_graph_build
    would capture them (same arguments and preconditions) and checks that every
    model yields the same kernels and grids (same grid, precision, closure and
-   options); the models' own state, tables and probe rings are used in place.
+   options; each model at most once); the models' own state, tables and probe
+   rings are used in place.
    hy_batch_launch enqueues nlaunch passes of those nsteps steps on the
    batch's stream and returns; hy_batch_idle / hy_batch_sync poll / wait.
    While a batch is in flight its models must not be used otherwise; a model

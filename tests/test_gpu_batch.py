@@ -86,6 +86,16 @@ def test_batch_rejects_mixed_shapes(gpu):
 
 
 @pytest.mark.gpu
+def test_batch_rejects_a_model_twice(gpu):
+    cache = serve._hnc.StructureCache()
+    m = serve.make_model(_reqs(1, closure="newtonian")[0], "fp64", cache)
+    with pytest.raises(abi.HydftError) as e:
+        Batch([m, m], 20, dt_every=20)
+    assert e.value.code == abi.HY_ERR_INVALID
+    m.close()
+
+
+@pytest.mark.gpu
 def test_batched_server_matches_alone(gpu):
     """The batching server groups same-shape requests; each result equals
     the one-at-a-time result (FP64, bitwise)."""
