@@ -33,6 +33,29 @@ def test_fft_known_answers_1d(Spec):
     assert abs(s.mode(u, 3, 0, 0) - 0.15) < 1e-13
 
 
+def test_fft_golden_known_answers(Spec):
+    """The test_fft.f90 known answers as committed data
+    (tests/golden/reference_known_answers.json), through the GPU transforms."""
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_known_answers.json")))["fft"]
+    c = g["one_d"]
+    s = Spec(c["n"], 1, 1, c["length"], TWOPI, TWOPI)
+    x = np.arange(c["n"]) * (c["length"] / c["n"])
+    u = (1.5 + 0.3 * np.cos(3 * x) + 0.2 * np.sin(5 * x)).reshape(1, 1, c["n"])
+    uh = s.fwd(u)
+    assert np.abs(u - s.bwd(uh)).max() < g["round_trip_tol"]
+    for co in c["coefficients"]:
+        assert abs(uh[0, 0, co["kx"]] - complex(co["re"], co["im"])) < c["tol"]
+    c = g["three_d"]
+    s = Spec(*c["shape_xyz"], *c["lengths_xyz"])
+    X, Y, Z = O.Grid(*c["shape_xyz"], *c["lengths_xyz"]).coords()
+    u = 0.7 + np.cos(2 * X) * np.sin(TWOPI * Y / 4.0) + 0.4 * np.cos(TWOPI * 2.0 * Z)
+    assert np.abs(u - s.bwd(s.fwd(u))).max() < g["round_trip_tol"]
+    for mo in c["modes"]:
+        assert abs(s.mode(u, *mo["m"]) - complex(mo["re"], mo["im"])) < c["tol"]
+
+
 def test_fft_known_answers_3d(Spec):
     """test_fft.f90:38-52."""
     s = Spec(16, 12, 8, TWOPI, 4.0, 1.0)

@@ -81,3 +81,30 @@ def test_library_host_fd_matches_oracle():
         np.testing.assert_allclose(fd_eval(which, xs), f(xs), rtol=1e-13, atol=0)
     ys = fd.ip12(xs)
     np.testing.assert_allclose(fd_eval(7, ys), fd.alpha_from_ip12(ys), rtol=1e-12, atol=1e-12)
+
+
+def test_golden_known_answers():
+    """The known answers of test_fermi_dirac.f90 as committed data
+    (tests/golden/reference_known_answers.json), against the oracle and
+    libhydft's host implementation (hy_fd_eval, the code the kernels use)."""
+    import json
+    import os
+    from hydft_amd.model import fd_eval, FD_IM12, FD_IP12, FD_ALPHA
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_known_answers.json")))["fermi_dirac"]
+    kv = g["known_value"]
+    assert abs(v(fd.ip12, kv["alpha"]) - kv["value"]) < kv["abs_tol"]
+    assert abs(float(fd_eval(FD_IP12, kv["alpha"])[0]) - kv["value"]) < kv["abs_tol"]
+    cl = g["classical_limit"]
+    for name, f, code in (("im12", fd.im12, FD_IM12), ("ip12", fd.ip12, FD_IP12)):
+        ex = cl["gamma_p_plus_1"][name] * math.exp(cl["alpha"])
+        assert abs(v(f, cl["alpha"]) / ex - 1) < cl["rel_tol"]
+        assert abs(float(fd_eval(code, cl["alpha"])[0]) / ex - 1) < cl["rel_tol"]
+    dl = g["degenerate_limit"]
+    assert abs(v(fd.ip12, dl["alpha"]) / dl["ip12"] - 1) < dl["rel_tol"]
+    assert abs(v(fd.im12, dl["alpha"]) / dl["im12"] - 1) < dl["rel_tol"]
+    assert abs(float(fd_eval(FD_IP12, dl["alpha"])[0]) / dl["ip12"] - 1) < dl["rel_tol"]
+    for a in g["inverse"]["alphas"]:
+        y = v(fd.ip12, a)
+        tol = g["inverse"]["rel_tol"] * max(1.0, abs(a))
+        assert abs(v(fd.alpha_from_ip12, y) - a) < tol
+        assert abs(float(fd_eval(FD_ALPHA, y)[0]) - a) < tol

@@ -77,3 +77,28 @@ def test_mask_is_two_thirds_box():
     g = O.Grid(16, 8, 1, 1.0, 1.0, 1.0, dealias=False)
     assert g.mask[:, :, -1].sum() == 0 and g.mask[:, 4, :].sum() == 0
     assert g.mask.sum() == 8 * 7
+
+
+def test_golden_fft_known_answers():
+    """The known answers of test_fft.f90 as committed data
+    (tests/golden/reference_known_answers.json) against the oracle's r2c."""
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_known_answers.json")))["fft"]
+    c = g["one_d"]
+    s = O.Spec(c["n"], 1, 1, c["length"], TWOPI, TWOPI)
+    x = s.g.x
+    u = (1.5 + 0.3 * np.cos(3 * x) + 0.2 * np.sin(5 * x)).reshape(1, 1, c["n"])
+    uh = s.fwd(u)
+    assert np.abs(u - s.bwd(uh)).max() < g["round_trip_tol"]
+    for co in c["coefficients"]:
+        assert abs(uh[0, 0, co["kx"]] - complex(co["re"], co["im"])) < c["tol"]
+    c = g["three_d"]
+    nx, ny, nz = c["shape_xyz"]
+    lx, ly, lz = c["lengths_xyz"]
+    s = O.Spec(nx, ny, nz, lx, ly, lz)
+    X, Y, Z = s.g.coords()
+    u = 0.7 + np.cos(2 * X) * np.sin(TWOPI * Y / 4.0) + 0.4 * np.cos(TWOPI * 2.0 * Z)
+    assert np.abs(u - s.bwd(s.fwd(u))).max() < g["round_trip_tol"]
+    for mo in c["modes"]:
+        assert abs(s.mode(u, *mo["m"]) - complex(mo["re"], mo["im"])) < c["tol"]
