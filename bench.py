@@ -199,19 +199,23 @@ def serve_sample(nreq, steps, grids=(32, 48, 64)):
     t0 = time.perf_counter()
     cache.solve_many([hnc_request(r.p) for r in reqs])
     t_hnc = time.perf_counter() - t0
+    print("# serve sample: hnc presolve done", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     one = serve.one_at_a_time(reqs, precision="mixed", cache=cache)
     w_one = time.perf_counter() - t0
+    print("# serve sample: one at a time done", file=sys.stderr, flush=True)
     # concurrent runs, one stream and one captured graph per run
     srv_st = serve.Server(precision="mixed", cache=cache, max_slots=64)
     t0 = time.perf_counter()
     streams = srv_st.serve(reqs, realtime=False)
     w_streams = time.perf_counter() - t0
+    print("# serve sample: streams done", file=sys.stderr, flush=True)
     # batches: same-shape runs share every launch
     srv = serve.Server(precision="mixed", cache=cache, max_slots=64, batch=True)
     t0 = time.perf_counter()
     batch = srv.serve(reqs, realtime=False)
     w_batch = time.perf_counter() - t0
+    print("# serve sample: batches done", file=sys.stderr, flush=True)
     # served stream: Poisson arrivals, batches formed from what has arrived
     rate = 50.0
     reqs_rt = serve.chi_requests(nreq, seed=11, grid=grids, steps=steps, arrival_rate=rate)
@@ -219,6 +223,7 @@ def serve_sample(nreq, steps, grids=(32, 48, 64)):
     t0 = time.perf_counter()
     served = srv_rt.serve(reqs_rt, realtime=True)
     w_served = time.perf_counter() - t0
+    print("# serve sample: served stream done", file=sys.stderr, flush=True)
 
     # cold c(k) cache, end to end: every request needs its own HNC solve.  The
     # reference's way is a host solve then the run, one request at a time
@@ -229,10 +234,12 @@ def serve_sample(nreq, steps, grids=(32, 48, 64)):
     t0 = time.perf_counter()
     one_cold = serve.one_at_a_time(reqs[:ncold], precision="mixed", cache=cache_h)
     w_one_cold = (time.perf_counter() - t0) / ncold
+    print("# serve sample: cold one at a time done", file=sys.stderr, flush=True)
     srv_c = serve.Server(precision="mixed", cache=hnc.StructureCache(), max_slots=64, batch=True, hnc_device=True)
     t0 = time.perf_counter()
     served_cold = srv_c.serve(reqs, realtime=False)
     w_cold = time.perf_counter() - t0
+    print("# serve sample: cold served done", file=sys.stderr, flush=True)
     cold_by = {r.rid: r for r in served_cold}
     chi_diff = max(abs(cold_by[r.rid].value["chi"] / r.value["chi"] - 1.0) for r in one_cold)
 

@@ -124,6 +124,7 @@ _SIGS = {
     "hy_batch_sync": [_P],
     "hy_batch_destroy": [_P],
     "hy_model_probe_ring": [_P, _I, _P],
+    "hy_model_kept": [_P, _PI],
     "hy_hnc_iterate": [_I, _I, C.c_double, C.c_double, _P, _P, _P, _P, C.c_double, C.c_double, _I, _P, _P, _P, _P],
 }
 

@@ -2446,13 +2446,23 @@ int hy_batch_create(const hy_model* ms, int n, int nsteps, int dt_every, const i
   // every model must have produced the same sequence of launches
   const size_t L = recs[0].items.size();
   for (int i = 1; i < n; ++i) {
-    if (recs[i].items.size() != L) return set_error(HY_ERR_INVALID, "hy_batch_create: models differ in shape or options");
+    if (recs[i].items.size() != L)
+      return set_error(HY_ERR_INVALID, "hy_batch_create: models differ in shape or options (model " +
+                                           std::to_string(i) + ": " + std::to_string(recs[i].items.size()) +
+                                           " launches, model 0: " + std::to_string(L) + ")");
     for (size_t p = 0; p < L; ++p) {
       const LaunchRec& a = recs[0].items[p];
       const LaunchRec& c = recs[i].items[p];
       if (a.fn_batched != c.fn_batched || a.grid.x != c.grid.x || a.grid.y != c.grid.y || a.block.x != c.block.x ||
           a.xline != c.xline || a.args.size() != c.args.size())
-        return set_error(HY_ERR_INVALID, "hy_batch_create: models differ in shape or options");
+        return set_error(HY_ERR_INVALID, "hy_batch_create: models differ in shape or options (model " +
+                                             std::to_string(i) + ", launch " + std::to_string(p) + ": " +
+                                             (a.fn_batched != c.fn_batched ? "kernel" : "grid or arguments") +
+                                             ", grid " + std::to_string(a.grid.x) + "x" + std::to_string(a.grid.y) +
+                                             " vs " + std::to_string(c.grid.x) + "x" + std::to_string(c.grid.y) +
+                                             ", block " + std::to_string(a.block.x) + " vs " +
+                                             std::to_string(c.block.x) + ", args " + std::to_string(a.args.size()) +
+                                             " vs " + std::to_string(c.args.size()) + ")");
     }
   }
   // one device allocation: per launch position, n argument structs (or blocks)
@@ -2539,6 +2549,14 @@ int hy_batch_sync(hy_batch b) {
 
 int hy_batch_destroy(hy_batch b) {
   batch_free(b);
+  return HY_OK;
+}
+
+int hy_model_kept(hy_model h, int* kept3) {
+  if (!h || !kept3) return set_error(HY_ERR_INVALID, "hy_model_kept: null argument");
+  kept3[0] = h->m.g.Kx;
+  kept3[1] = h->m.g.Ky;
+  kept3[2] = h->m.g.Kz;
   return HY_OK;
 }
 

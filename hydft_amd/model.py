@@ -311,6 +311,12 @@ class Model:
         abi.call("hy_model_device_bytes", self.h, C.byref(v))
         return int(v.value)
 
+    def kept(self):
+        """The kept box (Kx, Ky, Kz) of the model's grid."""
+        k = (C.c_int * 3)()
+        abi.call("hy_model_kept", self.h, k)
+        return tuple(int(v) for v in k)
+
     def probe_ring(self, ring):
         """`ring` rows of the device probe ring (slot (s-1) % ring for step s)."""
         buf = np.zeros((int(ring), 6))

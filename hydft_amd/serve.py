@@ -21,7 +21,9 @@ what else runs beside it.
 
 With `batch=True` the server admits the arrived requests in groups of the
 same shape: same grid, species kind, closure, drive kind, probed mode, time
-step policy and step count.  A group runs as one batch (hy_batch_*): each
+step policy and step count, split further by the models' kept box (for N
+divisible by 3 the 2/3-rule boundary mode depends on the box length's
+rounding).  A group runs as one batch (hy_batch_*): each
 kernel of a step is launched once for all its models (grid z = model), so a
 step of B small runs costs the launches of one.  A batch is one slot; its
 models' device bytes are charged to the budget as a whole.  A group the
@@ -210,24 +212,32 @@ class Server:
         models = [make_model(r, self.precision, self.cache) for r in reqs]
         # each run is charged the larger of the estimate it was admitted on and its actual bytes
         charge = [max(m.device_bytes(), estimate_bytes(r.p, self.precision)) for m, r in zip(models, reqs)]
-        if len(reqs) > 1:
-            from .model import Batch
-            from . import abi
-            try:
-                b = Batch(models, self.chunk, dt_every=dt_every, probe=req.mode, ring=steps)
-            except abi.HydftError as e:
-                if e.code != abi.HY_ERR_INVALID:
-                    raise
-            else:
-                b.launch(launches)
-                self.batches_launched += 1
-                return [_Slot(list(reqs), models, now, sum(charge), steps, b)]
-        out = []
+        # same-key runs can still differ in their kept box (a 2/3-rule boundary
+        # mode decided by the box length's rounding): one batch per kept box
+        parts = {}
         for r, m, c in zip(reqs, models, charge):
-            m.graph_build(self.chunk, dt_every=dt_every, probe=r.mode, ring=steps)
-            m.graph_launch(launches)
-            self.batches_launched += 1
-            out.append(_Slot([r], [m], now, c, steps))
+            parts.setdefault(m.kept(), []).append((r, m, c))
+        out = []
+        for part in parts.values():
+            if len(part) > 1:
+                from .model import Batch
+                from . import abi
+                ms = [m for _, m, _ in part]
+                try:
+                    b = Batch(ms, self.chunk, dt_every=dt_every, probe=req.mode, ring=steps)
+                except abi.HydftError as e:
+                    if e.code != abi.HY_ERR_INVALID:
+                        raise
+                else:
+                    b.launch(launches)
+                    self.batches_launched += 1
+                    out.append(_Slot([r for r, _, _ in part], ms, now, sum(c for _, _, c in part), steps, b))
+                    continue
+            for r, m, c in part:
+                m.graph_build(self.chunk, dt_every=dt_every, probe=r.mode, ring=steps)
+                m.graph_launch(launches)
+                self.batches_launched += 1
+                out.append(_Slot([r], [m], now, c, steps))
         return out
 
     def _take(self, pending, now, realtime, room_bytes):

@@ -218,6 +218,12 @@ int hy_batch_idle(hy_batch b, int* idle);
 int hy_batch_sync(hy_batch b);
 int hy_batch_destroy(hy_batch b);
 int hy_model_probe_ring(hy_model m, int ring, double* h_probe);
+/* The kept box (Kx, Ky, Kz) of the model's grid (grid.f90:90-108).  Models
+   of the same nx, ny, nz can differ in it: when N is divisible by 3 the
+   boundary mode sits exactly on the 2/3-rule threshold and the comparison
+   depends on the box length's rounding; only models of the same kept box
+   share a batch. */
+int hy_model_kept(hy_model m, int* kept3);
 
 /* ---- batched OZ-HNC direct solve (oz_hnc.f90:111-140 direct branch, the
    iteration of oz_hnc.f90:142-296) --------------------------------------

@@ -110,3 +110,30 @@ def test_batched_server_matches_alone(gpu):
     for rid in a:
         assert np.array_equal(a[rid].probe, b[rid].probe), rid
         assert a[rid].value["chi"] == b[rid].value["chi"], rid
+
+
+@pytest.mark.gpu
+def test_batches_split_by_kept_box(gpu):
+    """48^3 runs whose box lengths put the 2/3-rule boundary mode on
+    different sides of the threshold have different kept boxes; the server
+    runs one batch per kept box, each result equal to its one-at-a-time run."""
+    reqs = [r for r in serve.chi_requests(200, seed=11, grid=(48,), steps=40) if r.p.closure == "newtonian"]
+    for r in reqs:
+        r.p.terms, r.p.structure = "ideal,hartree", "none"
+    boxes = {}
+    for r in reqs:
+        m = serve.make_model(r, "fp64")
+        boxes.setdefault(m.kept(), []).append(r)
+        m.close()
+        if len(boxes) >= 2 and all(len(v) >= 2 for v in boxes.values()):
+            break
+    assert len(boxes) >= 2, list(boxes)
+    pick = [v[i] for v in list(boxes.values())[:2] for i in range(2)]
+    for i, r in enumerate(pick):
+        r.rid = i
+    srv = serve.Server(precision="fp64", batch=True)
+    b = {r.rid: r for r in srv.serve(pick, realtime=False)}
+    assert srv.batches_launched == 2
+    a = {r.rid: r for r in serve.one_at_a_time(pick, precision="fp64")}
+    for rid in a:
+        assert np.array_equal(a[rid].probe, b[rid].probe), rid
