@@ -162,32 +162,40 @@ struct PencilArgs {
   PencilField f[MAXPF];
   int A, B, L, K, h;        // contiguous count, outer count, FFT length, stored count, #nonneg stored
   double scale;             // forward: 1/L
-  const double* kax;        // wavenumbers along the pass axis in FFT order (L entries)
+  // Wavenumbers are computed as (signed index) x spacing: a table load here
+  // would be a dependent load in front of each multiplied element.
+  double dkax;              // 2 pi / (box length) along the pass axis
   // decoding of the contiguous index for MUL_NEGK2 (first inverse pass only):
-  const double* kxv; int ax_x;   // kx array; x count in the contiguous index
-  const double* kyv; int ay_n;   // ky array (FFT order, ny entries); y stored count in the contiguous index (0 = none)
+  double dkx; int ax_x;     // kx spacing; x count in the contiguous index
+  double dky; int ay_n;     // ky spacing; y stored count in the contiguous index (0 = none)
   int ay_h, ay_L;                // y stored-index mapping (h, L) when ay_n > 0
   int y_from_pencil;             // 1: ky comes from the pencil position (y pass)
 };
 
+// signed index of FFT position p on an axis of length n (grid.f90:116-123)
+__device__ inline int wrap_index(int p, int n) { return p <= n / 2 ? p : p - n; }
+// Written without FMA contraction: the single and batched forms of a pencil
+// kernel then round every product and sum alike, so batched runs equal
+// single runs bit for bit.
 template <class T>
 __device__ inline cpx<T> apply_mul(cpx<T> v, int mul, int p, int ga, const PencilArgs& a) {
+#pragma clang fp contract(off)
   if (mul == MUL_IK) {
-    double k = a.kax[p];
+    double k = (double)wrap_index(p, a.L) * a.dkax;
     return mk<T>((T)(-k * (double)v.y), (T)(k * (double)v.x));
   }
   if (mul == MUL_NEGK2) {
     int ix = ga % a.ax_x;
-    double kx = a.kxv[ix];
+    double kx = (double)ix * a.dkx;
     double ky = 0.0, kz = 0.0;
     if (a.y_from_pencil) {
-      ky = a.kax[p];
+      ky = (double)wrap_index(p, a.L) * a.dkax;
     } else {
-      kz = a.kax[p];
+      kz = (double)wrap_index(p, a.L) * a.dkax;
       if (a.ay_n > 0) {
         int iy = ga / a.ax_x;
         int jy = iy < a.ay_h ? iy : iy + (a.ay_L - a.ay_n);
-        ky = a.kyv[jy];
+        ky = (double)wrap_index(jy, a.ay_L) * a.dky;
       }
     }
     double k2 = kx * kx + ky * ky + kz * kz;
@@ -196,10 +204,37 @@ __device__ inline cpx<T> apply_mul(cpx<T> v, int mul, int p, int ga, const Penci
   return v;
 }
 
+// Field and twiddle pointers are device (global) memory.  The batched kernels
+// read them from a device array, where the compiler cannot infer that and
+// would emit flat accesses; these helpers make every pencil access a global
+// one.  (A host-side build of this file uses plain pointers.)
+template <class T>
+__device__ inline cpx<T> gload_c(const void* p, size_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  const v2 v = ((const __attribute__((address_space(1))) v2*)p)[i];
+  return mk<T>(v.x, v.y);
+#else
+  return reinterpret_cast<const cpx<T>*>(p)[i];
+#endif
+}
+template <class T>
+__device__ inline void gstore_c(void* p, size_t i, cpx<T> c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  v2 v;
+  v.x = c.x;
+  v.y = c.y;
+  ((__attribute__((address_space(1))) v2*)p)[i] = v;
+#else
+  reinterpret_cast<cpx<T>*>(p)[i] = c;
+#endif
+}
+
 template <class T>
 __device__ inline cpx<T> load_any(const void* p, size_t i, int is64) {
-  if (is64) return cvt<T>(reinterpret_cast<const c64*>(p)[i]);
-  return cvt<T>(reinterpret_cast<const c32*>(p)[i]);
+  if (is64) return cvt<T>(gload_c<double>(p, i));
+  return cvt<T>(gload_c<float>(p, i));
 }
 
 // Block shape of the pencil kernels: NA contiguous columns (256 bytes of one
@@ -252,7 +287,7 @@ __device__ __attribute__((always_inline)) inline void pencil_inv_body(const Penc
     }
     return v;
   };
-  cpx<T>* out = reinterpret_cast<cpx<T>*>(f.out);
+  void* out = f.out;
   if constexpr (TPC == 1) {
     cpx<T> v[L];
 #pragma unroll
@@ -260,7 +295,7 @@ __device__ __attribute__((always_inline)) inline void pencil_inv_body(const Penc
     rdft<T, L, true>(v);
     if (live) {
 #pragma unroll
-      for (int p = 0; p < L; ++p) out[((size_t)b * L + p) * a.A + ga] = v[p];
+      for (int p = 0; p < L; ++p) gstore_c<T>(out, ((size_t)b * L + p) * a.A + ga, v[p]);
     }
   } else {
     __shared__ cpx<T> lds[L * NA];
@@ -274,7 +309,7 @@ __device__ __attribute__((always_inline)) inline void pencil_inv_body(const Penc
 #pragma unroll
       for (int k1 = 0; k1 < L1; ++k1) {
         cpx<T> w = v[k1];
-        if (k1 > 0) w = cmulc(w, twg[(n2 * k1) % L]);
+        if (k1 > 0) w = cmulc(w, gload_c<T>(twg, (n2 * k1) % L));
         lds[(k1 * L2 + n2) * NA + al] = w;
       }
     }
@@ -288,7 +323,7 @@ __device__ __attribute__((always_inline)) inline void pencil_inv_body(const Penc
       rdft<T, L2, true>(w);
       if (live) {
 #pragma unroll
-        for (int k2 = 0; k2 < L2; ++k2) out[((size_t)b * L + k1 + L1 * k2) * a.A + ga] = w[k2];
+        for (int k2 = 0; k2 < L2; ++k2) gstore_c<T>(out, ((size_t)b * L + k1 + L1 * k2) * a.A + ga, w[k2]);
       }
     }
   }
@@ -314,10 +349,10 @@ __device__ __attribute__((always_inline)) inline void pencil_fwd_body(const Penc
   auto load = [&](int p) -> cpx<T> {
     return live ? load_any<T>(f.in, ((size_t)b * L + p) * a.A + ga, f.in_f64) : mk<T>(0, 0);
   };
-  cpx<T>* out = reinterpret_cast<cpx<T>*>(f.out);
+  void* out = f.out;
   auto store = [&](int p, cpx<T> v) {
     const int t = kept_index(p, a.h, tailoff);
-    if (t >= 0 && live) out[((size_t)b * a.K + t) * a.A + ga] = apply_mul<T>(scal(v, sc), f.mul, p, ga, a);
+    if (t >= 0 && live) gstore_c<T>(out, ((size_t)b * a.K + t) * a.A + ga, apply_mul<T>(scal(v, sc), f.mul, p, ga, a));
   };
   if constexpr (TPC == 1) {
     cpx<T> v[L];
@@ -338,7 +373,7 @@ __device__ __attribute__((always_inline)) inline void pencil_fwd_body(const Penc
 #pragma unroll
       for (int k1 = 0; k1 < L1; ++k1) {
         cpx<T> w = v[k1];
-        if (k1 > 0) w = cmul(w, twg[(n2 * k1) % L]);
+        if (k1 > 0) w = cmul(w, gload_c<T>(twg, (n2 * k1) % L));
         lds[(k1 * L2 + n2) * NA + al] = w;
       }
     }
@@ -415,10 +450,11 @@ int pencil_pass(const Grid& g, int axis, bool inverse, bool trunc, int A, int B,
   a.K = trunc ? (axis == 1 ? g.Ky : g.Kz) : a.L;
   a.h = trunc ? (a.K + 1) / 2 : a.L;
   a.scale = 1.0 / a.L;
-  a.kax = axis == 1 ? g.d_ky : g.d_kz;
-  a.kxv = g.d_kx;
+  const double twopi = 2.0 * 3.14159265358979323846264338327950288;
+  a.dkax = twopi / (axis == 1 ? g.ly : g.lz);
+  a.dkx = twopi / g.lx;
   a.ax_x = trunc ? g.Kx : g.nkx;
-  a.kyv = g.d_ky;
+  a.dky = twopi / g.ly;
   a.y_from_pencil = axis == 1 ? 1 : 0;
   if (axis == 2 && g.ny > 1) {
     a.ay_n = trunc ? g.Ky : g.ny;
