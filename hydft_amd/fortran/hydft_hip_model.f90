@@ -158,6 +158,13 @@ module hydft_hip_model
       type(c_ptr), value :: m, probe
       integer(c_int), value :: ring
     end function
+    !> kept 2/3-rule box (Kx, Ky, Kz) of a model: models batch together only
+    !> when their boxes agree
+    integer(c_int) function hy_model_kept(m, kept3) bind(C, name='hy_model_kept')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int), intent(out) :: kept3(3)
+    end function
     ! batched OZ-HNC direct solve on the GPU (oz_hnc.f90:111-296 for many state points)
     integer(c_int) function hy_hnc_iterate(nsolve, nr, dr, dk, n0, bvs, bvlk, gs0, tol, mix, maxiter, gs, status, &
         niter, resid) bind(C, name='hy_hnc_iterate')
