@@ -5,8 +5,8 @@
 !> and Hartree terms and no viscosity, so that linear theory is the closed
 !> form omega = q sqrt(n0 K(q) / m), K(q) = 1/(beta n0) + 4 pi e2/(q^2 + kappa^2).
 !> Checks: frequency within 1 % of linear theory, mass conserved to 1e-12
-!> (test_conservation.f90), and batched and captured steps equal to
-!> hy_model_advance bitwise.
+!> (test_conservation.f90), the kept 2/3-rule box, and batched and captured
+!> steps equal to hy_model_advance bitwise.
 program drv_model
   use, intrinsic :: iso_c_binding
   use hydft_hip_c
@@ -24,7 +24,7 @@ program drv_model
   real(c_double), target :: n(nx), mom(nx, 3)
   real(c_double) :: n0, beta, e2, q, kq, om_lin, om, t, dt, bud0(6), bud1(6), err
   real(c_double) :: tz(2000)
-  integer(c_int) :: taken, step, idle
+  integer(c_int) :: taken, step, idle, kept(3)
   integer :: i, nz, nfail, nsteps
 
   nfail = 0
@@ -46,6 +46,9 @@ program drv_model
 
   ! ---- single mode vs linear theory
   call hy_check(hy_model_create(desc, m), 'model_create')
+  ! kept 2/3-rule box: |kx| < (2/3) (2 pi / lx) (nx / 2) keeps kx indices 0..10 of nx = 32
+  call hy_check(hy_model_kept(m, kept), 'model_kept')
+  call check(all(kept == [11, 1, 1]), 'kept box', real(kept(1), c_double))
   call set_mode(m, 1.0e-4_c_double)
   call hy_check(hy_model_budgets(m, bud0), 'budgets')
   nsteps = 1000
