@@ -147,9 +147,10 @@ __global__ void k_ctl_b(const CtlK* __restrict__ ks) { ctl_body(ks[blockIdx.z]);
 // emulation, uses plain pointers.)
 // Loads and stores through the global address space with a 32-bit byte
 // offset from a uniform base.  The x-line kernels take their buffer pointers
-// from an argument block in memory, which the compiler cannot place in an
-// address space; this form gives scalar-base global accesses instead of
-// 64-bit flat addresses per lane.  Every buffer addressed this way is below
+// from an argument block in memory, and the batched K1/K2 kernels from a
+// device array; the compiler cannot place such pointers in an address space,
+// and this form gives scalar-base global accesses instead of 64-bit flat
+// addresses per lane.  Every buffer addressed this way is below
 // 4 GiB.
 template <class V>
 __device__ inline V gld(const void* base, size_t elem) {
@@ -780,9 +781,11 @@ __device__ inline void spec_k(const SpecArgs& a, size_t e, double k[3]) {
   k[1] = sy * a.dky;
   k[2] = sz * a.dkz;
 }
+// K1/K2 field accesses go through gld/gst (global, not flat, in the batched
+// kernels, whose pointers come from a device array)
 template <class T>
 __device__ inline c64 ld(const void* p, size_t i) {
-  return cvt<double>(reinterpret_cast<const cpx<T>*>(p)[i]);
+  return cvt<double>(gldc<T>(p, i));
 }
 
 // Newtonian stress Pi0 from u_hat (stress.f90:76-82) evaluated spectrally
@@ -816,28 +819,28 @@ template <class T>
 __device__ __attribute__((always_inline)) inline void spec1_body(const K1Args& a) {
   const size_t nt = a.s.nt;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nt; e += (size_t)gridDim.x * blockDim.x) {
-    c64 nh = a.N[e];
+    c64 nh = gldc<double>(a.N, e);
     if (e == 0) nh.x -= a.n0;  // fe_correlation_ry.f90:78
-    c64 mu = ld<T>(a.fmu, e) + scal(nh, a.kcomb[e]);
+    c64 mu = ld<T>(a.fmu, e) + scal(nh, gld<double>(a.kcomb, e));
     if (e == 0) mu = mk<double>(0, 0);  // only grad(mu) is used
-    reinterpret_cast<cpx<T>*>(a.muhat)[e] = cvt<T>(mu);
+    gstc<T>(a.muhat, e, cvt<T>(mu));
     if (a.evolve) {
       double k[3];
       spec_k(a.s, e, k);
       c64 un[3];
       for (int d = 0; d < 3; ++d) un[d] = d < a.s.nd ? ld<T>(a.fu[d], e) : mk<double>(0, 0);
       if (a.relax) {  // model.f90:209-214, applied with this stage's u = u after the update
-        c64 uo[3] = {a.uold[e], a.uold[nt + e], a.uold[2 * nt + e]};
+        c64 uo[3] = {gldc<double>(a.uold, e), gldc<double>(a.uold, nt + e), gldc<double>(a.uold, 2 * nt + e)};
         c64 p0n[6], p0o[6];
         pi0_hat(un, k, a.eta, a.xi, p0n);
         pi0_hat(uo, k, a.eta, a.xi, p0o);
         const double fac = a.ctl->relax_fac, phi = a.ctl->relax_phi;
         for (int c = 0; c < 6; ++c) {
-          c64 pc = a.P[c * nt + e];
-          a.P[c * nt + e] = p0n[c] - scal(p0n[c] - p0o[c], phi) + scal(pc - p0o[c], fac);
+          c64 pc = gldc<double>(a.P, c * nt + e);
+          gstc<double>(a.P, c * nt + e, p0n[c] - scal(p0n[c] - p0o[c], phi) + scal(pc - p0o[c], fac));
         }
       }
-      for (int d = 0; d < 3; ++d) a.uold[d * nt + e] = un[d];
+      for (int d = 0; d < 3; ++d) gstc<double>(a.uold, d * nt + e, un[d]);
     }
   }
 }
@@ -877,7 +880,7 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
     // continuity: dn = -div(mom)/m (model.f90:95-96)
     c64 km = mk<double>(0, 0);
 #pragma unroll
-    for (int d = 0; d < ND; ++d) km = km + scal(a.S[(1 + d) * nt + e], k[d]);
+    for (int d = 0; d < ND; ++d) km = km + scal(gldc<double>(a.S, (1 + d) * nt + e), k[d]);
     c64 r[10];
     r[0] = mk<double>(km.y * a.inv_mass, -km.x * a.inv_mass);
     // stress divergence f_i = d_j Pi_ij
@@ -887,7 +890,7 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
     const bool have_pi = a.evolve || a.stress_newt;
     if (a.evolve) {
 #pragma unroll
-      for (int c = 0; c < 6; ++c) pih[c] = a.S[(4 + c) * nt + e];
+      for (int c = 0; c < 6; ++c) pih[c] = gldc<double>(a.S, (4 + c) * nt + e);
     } else if (a.stress_newt) {
       c64 u[3];
 #pragma unroll
@@ -925,18 +928,18 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
     }
     auto put = [&](int f, c64 rf) {
       if (a.rhs_only) {
-        a.R[f * nt + e] = rf;
+        gstc<double>(a.R, f * nt + e, rf);
         return;
       }
-      c64 sv = a.S[f * nt + e];
+      c64 sv = gldc<double>(a.S, f * nt + e);
       if (a.stage == 0) {
         sv = sv + scal(rf, dt * ra);
       } else {
-        const c64 ro = a.R[f * nt + e];
+        const c64 ro = gldc<double>(a.R, f * nt + e);
         sv = sv + scal(scal(rf, ra) + scal(ro, rb), dt);
       }
-      a.S[f * nt + e] = sv;
-      a.R[f * nt + e] = rf;
+      gstc<double>(a.S, f * nt + e, sv);
+      gstc<double>(a.R, f * nt + e, rf);
     };
 #pragma unroll
     for (int f = 0; f < 4; ++f) put(f, r[f]);
