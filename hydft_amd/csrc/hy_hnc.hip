@@ -3,7 +3,8 @@
 // for many state points at once.  The radial Fourier-Bessel pair is a DST-I
 // of length nr (N + 1 = 8193 = 3 * 2731 at the default nr: no short FFT), so
 // each transform of all active solves is one FP64 matrix product with the
-// DST-I matrix, held in HBM (rocBLAS dgemm on the matrix cores).  The
+// DST-I matrix, held in HBM (streaming passes; rocBLAS dgemm on the matrix
+// cores for more than DST_STREAM_MAX active solves).  The
 // pointwise closure, the OZ relation, the residual and the Ng dot products
 // are HIP kernels with one workgroup per solve.  Every solve keeps its own
 // control state on the host (iteration count, Ng history length, cooldown,
@@ -40,12 +41,14 @@ struct Dims {
   double dr, dk, cap;
 };
 
-// Y[:, p] = S X[:, p] for p < na <= NA_MAX (few active solves: the matrix
-// is streamed once, which a library GEMM with so few columns does at a
-// fraction of the bandwidth).  Block (row tile, chunk): thread = row j, the
-// chunk's X rows staged in LDS; per-chunk partial sums, added in a fixed
-// order by k_dst_sum (deterministic).
-constexpr int NA_MAX = 16, DCH = 32, DTILE = 256, DSTAGE = 256;
+// Y[:, p] = S X[:, p] for p < na <= NA_MAX: one streaming pass over the
+// matrix.  Up to DST_STREAM_MAX active solves take ceil(na / NA_MAX) such
+// passes; a library GEMM with so few columns keeps only M/128 workgroups busy
+// (65 at nr = 8193) and is 2-7x slower.  Block (row tile, chunk): thread = row
+// j, the chunk's X rows staged in LDS; per-chunk partial sums, added in a
+// fixed order by k_dst_sum (deterministic).
+constexpr int NA_MAX = 32, DCH = 32, DTILE = 256, DSTAGE = 256;
+constexpr int DST_STREAM_MAX = 192;
 template <int NA>
 __global__ __launch_bounds__(DTILE) void k_dst_few(const double* __restrict__ S, const double* __restrict__ X,
                                                    double* __restrict__ part, int nr, int na) {
@@ -53,14 +56,15 @@ __global__ __launch_bounds__(DTILE) void k_dst_few(const double* __restrict__ S,
   const int c = blockIdx.y;
   const int clen = (nr + DCH - 1) / DCH;
   const int m0 = c * clen, m1 = min(nr, m0 + clen);
-  __shared__ double xs[DSTAGE][NA];
+  constexpr int ST = NA <= 16 ? DSTAGE : DSTAGE * 16 / NA;  // X rows staged: <= 32 KB of LDS
+  __shared__ double xs[ST][NA];
   double acc[NA];
 #pragma unroll
   for (int p = 0; p < NA; ++p) acc[p] = 0.0;
-  for (int mb = m0; mb < m1; mb += DSTAGE) {
-    const int mn = min(DSTAGE, m1 - mb);
+  for (int mb = m0; mb < m1; mb += ST) {
+    const int mn = min(ST, m1 - mb);
     __syncthreads();
-    for (int e = threadIdx.x; e < DSTAGE * NA; e += DTILE) {
+    for (int e = threadIdx.x; e < ST * NA; e += DTILE) {
       const int mm = e / NA, p = e % NA;
       xs[mm][p] = (mm < mn && p < na) ? X[(size_t)p * nr + mb + mm] : 0.0;
     }
@@ -340,18 +344,24 @@ int hy_hnc_iterate(int nsolve, int nr, double dr, double dk, const double* h_n0,
   for (int i = 0; i < nsolve; ++i) h_all[i] = i;
   if (hipMemcpy(all, h_all.data(), nsolve * 4, hipMemcpyHostToDevice) != hipSuccess) return fail(HY_ERR_HIP, "hnc");
   const double one = 1.0, zero = 0.0;
-  // Y = S X over the na active columns: matrix cores for many, a single
-  // streaming pass over S for few
+  // Y = S X over the na active columns: streaming passes over S of up to
+  // NA_MAX columns each for up to DST_STREAM_MAX columns, matrix cores beyond
   auto dst = [&](int na) -> bool {
-    if (na <= NA_MAX) {
+    if (na <= DST_STREAM_MAX) {
       const dim3 g((nr + DTILE - 1) / DTILE, DCH);
-      if (na <= 1) hipLaunchKernelGGL(k_dst_few<1>, g, dim3(DTILE), 0, st, S, X, part, nr, na);
-      else if (na <= 2) hipLaunchKernelGGL(k_dst_few<2>, g, dim3(DTILE), 0, st, S, X, part, nr, na);
-      else if (na <= 4) hipLaunchKernelGGL(k_dst_few<4>, g, dim3(DTILE), 0, st, S, X, part, nr, na);
-      else if (na <= 8) hipLaunchKernelGGL(k_dst_few<8>, g, dim3(DTILE), 0, st, S, X, part, nr, na);
-      else hipLaunchKernelGGL(k_dst_few<16>, g, dim3(DTILE), 0, st, S, X, part, nr, na);
-      hipLaunchKernelGGL(k_dst_sum, dim3(std::min<size_t>(((size_t)na * nr + 255) / 256, 4096)), dim3(256), 0, st,
-                         part, Y, nr, na);
+      for (int p0 = 0; p0 < na; p0 += NA_MAX) {
+        const int nap = std::min(NA_MAX, na - p0);
+        const double* Xp = X + (size_t)p0 * nr;
+        double* Yp = Y + (size_t)p0 * nr;
+        if (nap <= 1) hipLaunchKernelGGL(k_dst_few<1>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
+        else if (nap <= 2) hipLaunchKernelGGL(k_dst_few<2>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
+        else if (nap <= 4) hipLaunchKernelGGL(k_dst_few<4>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
+        else if (nap <= 8) hipLaunchKernelGGL(k_dst_few<8>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
+        else if (nap <= 16) hipLaunchKernelGGL(k_dst_few<16>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
+        else hipLaunchKernelGGL(k_dst_few<32>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
+        hipLaunchKernelGGL(k_dst_sum, dim3(std::min<size_t>(((size_t)nap * nr + 255) / 256, 4096)), dim3(256), 0,
+                           st, part, Yp, nr, nap);
+      }
       return hipGetLastError() == hipSuccess;
     }
     return rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, nr, na, nr, &one, S, nr, X, nr, &zero,
