@@ -2,9 +2,10 @@
 // (Picard + Ng iteration on gamma_s = h - c_s, restarts with halved mixing)
 // for many state points at once.  The radial Fourier-Bessel pair is a DST-I
 // of length nr (N + 1 = 8193 = 3 * 2731 at the default nr: no short FFT), so
-// each transform of all active solves is one FP64 matrix product with the
-// DST-I matrix, held in HBM (streaming passes; rocBLAS dgemm on the matrix
-// cores for more than DST_STREAM_MAX active solves).  The
+// each transform of all active solves is one FP64 product with the DST-I
+// matrix: passes that rebuild the matrix entries from a table in LDS (or
+// stream the matrix from HBM for large nr), and rocBLAS dgemm on the matrix
+// cores for more than DST_STREAM_MAX active solves.  The
 // pointwise closure, the OZ relation, the residual and the Ng dot products
 // are HIP kernels with one workgroup per solve.  Every solve keeps its own
 // control state on the host (iteration count, Ng history length, cooldown,
@@ -36,14 +37,22 @@ __global__ void k_dst_matrix(double* __restrict__ S, int n) {
   }
 }
 
+// The matrix entry (j, m) is T[q], q = (j+1)(m+1) mod 2(n+1): the same
+// value from a table of 2(n+1) entries.
+__global__ void k_dst_table(double* __restrict__ T, int n) {
+  const int two = 2 * (n + 1);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < two; q += gridDim.x * blockDim.x)
+    T[q] = 2.0 * sin(PI_ * (double)q / (double)(n + 1));
+}
+
 struct Dims {
   int nr;
   double dr, dk, cap;
 };
 
 // Y[:, p] = S X[:, p] for p < na <= NA_MAX: one streaming pass over the
-// matrix.  Up to DST_STREAM_MAX active solves take ceil(na / NA_MAX) such
-// passes; a library GEMM with so few columns keeps only M/128 workgroups busy
+// matrix (used when its table does not fit in LDS, see k_dst_tab).  Up to
+// DST_STREAM_MAX active solves take ceil(na / NA_MAX) such passes; a library GEMM with so few columns keeps only M/128 workgroups busy
 // (65 at nr = 8193) and is 2-7x slower.  Block (row tile, chunk): thread = row
 // j, the chunk's X rows staged in LDS; per-chunk partial sums, added in a
 // fixed order by k_dst_sum (deterministic).
@@ -92,12 +101,60 @@ __global__ __launch_bounds__(DTILE) void k_dst_few(const double* __restrict__ S,
       if (p < na) part[((size_t)c * NA_MAX + p) * nr + j] = acc[p];
   }
 }
-__global__ void k_dst_sum(const double* __restrict__ part, double* __restrict__ Y, int nr, int na) {
+// The same product without reading the matrix: the workgroup holds the
+// 2(n+1)-entry table in LDS and walks row j's entries q = (j+1)(m+1) mod
+// 2(n+1) by adding j+1 per column.  One row per thread; grid (row tiles,
+// chunks), per-chunk partial sums as in k_dst_few.  Needs 2(n+1) <= TAB_MAX.
+constexpr int TAB_MAX = 2 * (8192 + 1), TTILE = 1024;
+template <int NA>
+__global__ __launch_bounds__(TTILE) void k_dst_tab(const double* __restrict__ T, const double* __restrict__ X,
+                                                   double* __restrict__ part, int nr, int na, int nch) {
+  constexpr int ST = NA <= 8 ? 256 : 2048 / NA;  // X rows staged: <= 16 KB of LDS beside the table
+  __shared__ double tab[TAB_MAX];
+  __shared__ double xs[ST][NA];
+  const int two = 2 * (nr + 1);
+  for (int e = threadIdx.x; e < two; e += TTILE) tab[e] = T[e];
+  const int j = blockIdx.x * TTILE + threadIdx.x;
+  const int c = blockIdx.y;
+  const int clen = (nr + nch - 1) / nch;
+  const int m0 = c * clen, m1 = min(nr, m0 + clen);
+  const int step = j + 1;  // < two, so one conditional subtraction keeps q in range
+  int q = (int)(((long long)(j + 1) * (long long)(m0 + 1)) % two);
+  double acc[NA];
+#pragma unroll
+  for (int p = 0; p < NA; ++p) acc[p] = 0.0;
+  for (int mb = m0; mb < m1; mb += ST) {
+    const int mn = min(ST, m1 - mb);
+    __syncthreads();  // the table (first pass) and the previous stage
+    for (int e = threadIdx.x; e < ST * NA; e += TTILE) {
+      const int mm = e / NA, p = e % NA;
+      xs[mm][p] = (mm < mn && p < na) ? X[(size_t)p * nr + mb + mm] : 0.0;
+    }
+    __syncthreads();
+    if (j < nr) {
+      for (int mm = 0; mm < mn; ++mm) {
+        const double sv = tab[q];
+#pragma unroll
+        for (int p = 0; p < NA; ++p) acc[p] = fma(sv, xs[mm][p], acc[p]);
+        q += step;
+        if (q >= two) q -= two;
+      }
+    }
+  }
+  if (j < nr) {
+#pragma unroll
+    for (int p = 0; p < NA; ++p)
+      if (p < na) part[((size_t)c * NA_MAX + p) * nr + j] = acc[p];
+  }
+}
+
+// Y[:, p] = sum over the nch chunks of the partial sums, in chunk order
+__global__ void k_dst_sum(const double* __restrict__ part, double* __restrict__ Y, int nr, int na, int nch) {
   const size_t n = (size_t)na * nr;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     const int p = (int)(e / nr), j = (int)(e % nr);
     double v = 0.0;
-    for (int c = 0; c < DCH; ++c) v += part[((size_t)c * NA_MAX + p) * nr + j];
+    for (int c = 0; c < nch; ++c) v += part[((size_t)c * NA_MAX + p) * nr + j];
     Y[e] = v;
   }
 }
@@ -238,6 +295,7 @@ struct DstCache {
   std::mutex mu;
   int nr = 0;
   double* S = nullptr;
+  double* T = nullptr;  // its 2(nr+1) distinct entries
 };
 DstCache& dst_cache() {
   static DstCache c;
@@ -310,24 +368,29 @@ int hy_hnc_iterate(int nsolve, int nr, double dr, double dk, const double* h_n0,
   if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return fail(HY_ERR_HIP, "hnc: stream");
   if (rocblas_create_handle(&blas) != rocblas_status_success || rocblas_set_stream(blas, st) != rocblas_status_success)
     return fail(HY_ERR_HIP, "hnc: rocblas handle");
-  // the DST-I matrix for this nr
+  // the DST-I matrix (and its table) for this nr
   double* S = nullptr;
+  double* T = nullptr;
   {
     DstCache& c = dst_cache();
     std::lock_guard<std::mutex> lk(c.mu);
     if (c.nr != nr) {
-      if (c.S) {
+      if (c.S || c.T) {
         (void)hipDeviceSynchronize();
-        (void)hipFree(c.S);
-        c.S = nullptr;
+        if (c.S) (void)hipFree(c.S);
+        if (c.T) (void)hipFree(c.T);
+        c.S = c.T = nullptr;
         c.nr = 0;
       }
       if (halloc((void**)&c.S, (size_t)nr * nr * 8) != HY_OK) return fail(HY_ERR_OOM, "hnc: DST matrix");
+      if (halloc((void**)&c.T, (size_t)2 * (nr + 1) * 8) != HY_OK) return fail(HY_ERR_OOM, "hnc: DST table");
       hipLaunchKernelGGL(k_dst_matrix, dim3(4096), dim3(256), 0, st, c.S, nr);
+      hipLaunchKernelGGL(k_dst_table, dim3(64), dim3(256), 0, st, c.T, nr);
       if (hipStreamSynchronize(st) != hipSuccess) return fail(HY_ERR_HIP, "hnc: DST matrix kernel");
       c.nr = nr;
     }
     S = c.S;
+    T = c.T;
   }
   if (hipMemcpy(bvs, h_bvs, vec * 8, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(bvlk, h_bvlk, vec * 8, hipMemcpyHostToDevice) != hipSuccess ||
@@ -348,19 +411,32 @@ int hy_hnc_iterate(int nsolve, int nr, double dr, double dk, const double* h_n0,
   // NA_MAX columns each for up to DST_STREAM_MAX columns, matrix cores beyond
   auto dst = [&](int na) -> bool {
     if (na <= DST_STREAM_MAX) {
-      const dim3 g((nr + DTILE - 1) / DTILE, DCH);
+      // matrix-free from the LDS table when it fits, else streaming S
+      const bool tab = 2 * (nr + 1) <= TAB_MAX;
+      const int tiles = tab ? (nr + TTILE - 1) / TTILE : (nr + DTILE - 1) / DTILE;
+      const int nch = tab ? std::max(1, std::min(DCH, 256 / tiles)) : DCH;  // about one workgroup per CU
+      const dim3 g(tiles, nch);
       for (int p0 = 0; p0 < na; p0 += NA_MAX) {
         const int nap = std::min(NA_MAX, na - p0);
         const double* Xp = X + (size_t)p0 * nr;
         double* Yp = Y + (size_t)p0 * nr;
-        if (nap <= 1) hipLaunchKernelGGL(k_dst_few<1>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
-        else if (nap <= 2) hipLaunchKernelGGL(k_dst_few<2>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
-        else if (nap <= 4) hipLaunchKernelGGL(k_dst_few<4>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
-        else if (nap <= 8) hipLaunchKernelGGL(k_dst_few<8>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
-        else if (nap <= 16) hipLaunchKernelGGL(k_dst_few<16>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
-        else hipLaunchKernelGGL(k_dst_few<32>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
+        if (tab) {
+          if (nap <= 1) hipLaunchKernelGGL(k_dst_tab<1>, g, dim3(TTILE), 0, st, T, Xp, part, nr, nap, nch);
+          else if (nap <= 2) hipLaunchKernelGGL(k_dst_tab<2>, g, dim3(TTILE), 0, st, T, Xp, part, nr, nap, nch);
+          else if (nap <= 4) hipLaunchKernelGGL(k_dst_tab<4>, g, dim3(TTILE), 0, st, T, Xp, part, nr, nap, nch);
+          else if (nap <= 8) hipLaunchKernelGGL(k_dst_tab<8>, g, dim3(TTILE), 0, st, T, Xp, part, nr, nap, nch);
+          else if (nap <= 16) hipLaunchKernelGGL(k_dst_tab<16>, g, dim3(TTILE), 0, st, T, Xp, part, nr, nap, nch);
+          else hipLaunchKernelGGL(k_dst_tab<32>, g, dim3(TTILE), 0, st, T, Xp, part, nr, nap, nch);
+        } else {
+          if (nap <= 1) hipLaunchKernelGGL(k_dst_few<1>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
+          else if (nap <= 2) hipLaunchKernelGGL(k_dst_few<2>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
+          else if (nap <= 4) hipLaunchKernelGGL(k_dst_few<4>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
+          else if (nap <= 8) hipLaunchKernelGGL(k_dst_few<8>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
+          else if (nap <= 16) hipLaunchKernelGGL(k_dst_few<16>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
+          else hipLaunchKernelGGL(k_dst_few<32>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
+        }
         hipLaunchKernelGGL(k_dst_sum, dim3(std::min<size_t>(((size_t)nap * nr + 255) / 256, 4096)), dim3(256), 0,
-                           st, part, Yp, nr, nap);
+                           st, part, Yp, nr, nap, nch);
       }
       return hipGetLastError() == hipSuccess;
     }
