@@ -103,48 +103,69 @@ __global__ __launch_bounds__(DTILE) void k_dst_few(const double* __restrict__ S,
 }
 // The same product without reading the matrix: the workgroup holds the
 // 2(n+1)-entry table in LDS and walks row j's entries q = (j+1)(m+1) mod
-// 2(n+1) by adding j+1 per column.  One row per thread; grid (row tiles,
-// chunks), per-chunk partial sums as in k_dst_few.  Needs 2(n+1) <= TAB_MAX.
+// 2(n+1) by adding j+1 per column.  A workgroup covers TTILE rows, R per
+// thread (R = 2 for wide passes: each staged X value then feeds two FMAs);
+// grid (row tiles, chunks), per-chunk partial sums as in k_dst_few.  Needs
+// 2(n+1) <= TAB_MAX.
 constexpr int TAB_MAX = 2 * (8192 + 1), TTILE = 1024;
 template <int NA>
-__global__ __launch_bounds__(TTILE) void k_dst_tab(const double* __restrict__ T, const double* __restrict__ X,
-                                                   double* __restrict__ part, int nr, int na, int nch) {
+constexpr int tab_rows() { return NA >= 32 ? 2 : 1; }
+template <int NA>
+__global__ __launch_bounds__(TTILE / tab_rows<NA>()) void k_dst_tab(const double* __restrict__ T,
+                                                                  const double* __restrict__ X,
+                                                                  double* __restrict__ part, int nr, int na, int nch) {
+  constexpr int R = tab_rows<NA>(), NT = TTILE / R;
   constexpr int ST = NA <= 8 ? 256 : 2048 / NA;  // X rows staged: <= 16 KB of LDS beside the table
   __shared__ double tab[TAB_MAX];
   __shared__ double xs[ST][NA];
   const int two = 2 * (nr + 1);
-  for (int e = threadIdx.x; e < two; e += TTILE) tab[e] = T[e];
-  const int j = blockIdx.x * TTILE + threadIdx.x;
+  for (int e = threadIdx.x; e < two; e += NT) tab[e] = T[e];
   const int c = blockIdx.y;
   const int clen = (nr + nch - 1) / nch;
   const int m0 = c * clen, m1 = min(nr, m0 + clen);
-  const int step = j + 1;  // < two, so one conditional subtraction keeps q in range
-  int q = (int)(((long long)(j + 1) * (long long)(m0 + 1)) % two);
-  double acc[NA];
+  int j[R], step[R], q[R];
 #pragma unroll
-  for (int p = 0; p < NA; ++p) acc[p] = 0.0;
+  for (int r = 0; r < R; ++r) {
+    j[r] = blockIdx.x * TTILE + r * NT + threadIdx.x;
+    step[r] = j[r] + 1;  // < two, so one conditional subtraction keeps q in range
+    q[r] = (int)(((long long)(j[r] + 1) * (long long)(m0 + 1)) % two);
+  }
+  double acc[R][NA];
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int p = 0; p < NA; ++p) acc[r][p] = 0.0;
   for (int mb = m0; mb < m1; mb += ST) {
     const int mn = min(ST, m1 - mb);
     __syncthreads();  // the table (first pass) and the previous stage
-    for (int e = threadIdx.x; e < ST * NA; e += TTILE) {
+    for (int e = threadIdx.x; e < ST * NA; e += NT) {
       const int mm = e / NA, p = e % NA;
       xs[mm][p] = (mm < mn && p < na) ? X[(size_t)p * nr + mb + mm] : 0.0;
     }
     __syncthreads();
-    if (j < nr) {
-      for (int mm = 0; mm < mn; ++mm) {
-        const double sv = tab[q];
+    for (int mm = 0; mm < mn; ++mm) {
+      double sv[R];
 #pragma unroll
-        for (int p = 0; p < NA; ++p) acc[p] = fma(sv, xs[mm][p], acc[p]);
-        q += step;
-        if (q >= two) q -= two;
+      for (int r = 0; r < R; ++r) {
+        sv[r] = tab[q[r]];
+        q[r] += step[r];
+        if (q[r] >= two) q[r] -= two;
+      }
+#pragma unroll
+      for (int p = 0; p < NA; ++p) {
+        const double x = xs[mm][p];
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r][p] = fma(sv[r], x, acc[r][p]);
       }
     }
   }
-  if (j < nr) {
 #pragma unroll
-    for (int p = 0; p < NA; ++p)
-      if (p < na) part[((size_t)c * NA_MAX + p) * nr + j] = acc[p];
+  for (int r = 0; r < R; ++r) {
+    if (j[r] < nr) {
+#pragma unroll
+      for (int p = 0; p < NA; ++p)
+        if (p < na) part[((size_t)c * NA_MAX + p) * nr + j[r]] = acc[r][p];
+    }
   }
 }
 
@@ -426,7 +447,7 @@ int hy_hnc_iterate(int nsolve, int nr, double dr, double dk, const double* h_n0,
           else if (nap <= 4) hipLaunchKernelGGL(k_dst_tab<4>, g, dim3(TTILE), 0, st, T, Xp, part, nr, nap, nch);
           else if (nap <= 8) hipLaunchKernelGGL(k_dst_tab<8>, g, dim3(TTILE), 0, st, T, Xp, part, nr, nap, nch);
           else if (nap <= 16) hipLaunchKernelGGL(k_dst_tab<16>, g, dim3(TTILE), 0, st, T, Xp, part, nr, nap, nch);
-          else hipLaunchKernelGGL(k_dst_tab<32>, g, dim3(TTILE), 0, st, T, Xp, part, nr, nap, nch);
+          else hipLaunchKernelGGL(k_dst_tab<32>, g, dim3(TTILE / tab_rows<32>()), 0, st, T, Xp, part, nr, nap, nch);
         } else {
           if (nap <= 1) hipLaunchKernelGGL(k_dst_few<1>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
           else if (nap <= 2) hipLaunchKernelGGL(k_dst_few<2>, g, dim3(DTILE), 0, st, S, Xp, part, nr, nap);
