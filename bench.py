@@ -346,6 +346,30 @@ def batch_only(spec, steps):
     return 0
 
 
+def hnc_only(nstate):
+    """Solve the c(k) of the first nstate state points of the driven-mode scan
+    together on the GPU (hy_hnc_iterate), twice (the first call also builds
+    the DST-I table), and print the wall time of each: the batched HNC alone,
+    for profiling."""
+    from hydft_amd import abi, serve
+    from hydft_amd.host import hnc
+    from hydft_amd.model import hnc_request
+    abi.call("hy_init", 0)
+    reqs = serve.chi_requests(nstate, seed=11, grid=(32, 48, 64), steps=400)
+    pts = [hnc_request(r.p) for r in reqs]
+    nr, rmax, tol, mix, maxiter = pts[0][2:7]
+    out = {"states": nstate, "nr": nr}
+    for rep in range(2):
+        t0 = time.perf_counter()
+        res = hnc.hnc_solve_many_device([p[0] for p in pts], [p[1] for p in pts], nr, rmax, tol, mix, maxiter)
+        out[f"wall_s_{rep}"] = time.perf_counter() - t0
+        its = sorted(r.niter for r in res)
+        out["iterations_p50"], out["iterations_max"] = its[len(its) // 2], its[-1]
+        out["converged"] = all(r.converged for r in res)
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -362,12 +386,16 @@ def main():
                     help="batched/served chi(k, w) sample size (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=400)
     ap.add_argument("--no-maxwell", action="store_true", help="skip the 128^3 Maxwell FP64/mixed sample")
+    ap.add_argument("--hnc-only", type=int, default=0, metavar="NSTATE",
+                    help="only solve NSTATE state points of the scan with the batched GPU HNC, and time it")
     ap.add_argument("--batch-only", default="", metavar="B,N,CLOSURE",
                     help="only step one batch of B driven-mode runs on N^3 (profiling)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
         return pmc_child(args)
+    if args.hnc_only:
+        return hnc_only(args.hnc_only)
     if args.batch_only:
         return batch_only(args.batch_only, args.serve_steps)
     if args.gpus != 1:

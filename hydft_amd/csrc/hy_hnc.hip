@@ -311,7 +311,8 @@ __global__ __launch_bounds__(HW) void k_apply(const int* __restrict__ all, Dims 
   }
 }
 
-// DST-I matrix of the last nr, kept for the life of the process
+// DST-I table and (when needed) matrix of the last nr, kept for the life of
+// the process
 struct DstCache {
   std::mutex mu;
   int nr = 0;
@@ -401,15 +402,26 @@ int hy_hnc_iterate(int nsolve, int nr, double dr, double dk, const double* h_n0,
         if (c.S) (void)hipFree(c.S);
         if (c.T) (void)hipFree(c.T);
         c.S = c.T = nullptr;
-        c.nr = 0;
       }
-      if (halloc((void**)&c.S, (size_t)nr * nr * 8) != HY_OK) return fail(HY_ERR_OOM, "hnc: DST matrix");
-      if (halloc((void**)&c.T, (size_t)2 * (nr + 1) * 8) != HY_OK) return fail(HY_ERR_OOM, "hnc: DST table");
-      hipLaunchKernelGGL(k_dst_matrix, dim3(4096), dim3(256), 0, st, c.S, nr);
-      hipLaunchKernelGGL(k_dst_table, dim3(64), dim3(256), 0, st, c.T, nr);
-      if (hipStreamSynchronize(st) != hipSuccess) return fail(HY_ERR_HIP, "hnc: DST matrix kernel");
       c.nr = nr;
     }
+    // the table serves every pass when it fits in LDS; the 8 nr^2-byte matrix
+    // is built only for the GEMM path (more than DST_STREAM_MAX solves) or
+    // when the table does not fit
+    const bool tab_fits = 2 * (nr + 1) <= TAB_MAX;
+    const bool need_s = !tab_fits || nsolve > DST_STREAM_MAX;
+    bool built = false;
+    if (tab_fits && !c.T) {
+      if (halloc((void**)&c.T, (size_t)2 * (nr + 1) * 8) != HY_OK) return fail(HY_ERR_OOM, "hnc: DST table");
+      hipLaunchKernelGGL(k_dst_table, dim3(64), dim3(256), 0, st, c.T, nr);
+      built = true;
+    }
+    if (need_s && !c.S) {
+      if (halloc((void**)&c.S, (size_t)nr * nr * 8) != HY_OK) return fail(HY_ERR_OOM, "hnc: DST matrix");
+      hipLaunchKernelGGL(k_dst_matrix, dim3(4096), dim3(256), 0, st, c.S, nr);
+      built = true;
+    }
+    if (built && hipStreamSynchronize(st) != hipSuccess) return fail(HY_ERR_HIP, "hnc: DST matrix kernel");
     S = c.S;
     T = c.T;
   }

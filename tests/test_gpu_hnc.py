@@ -65,3 +65,20 @@ def test_server_solves_ck_on_device(gpu):
         assert np.max(np.abs(pb[:, 0] - pa[:, 0])) <= 1e-12 * pa[-1, 0], rid
         assert np.max(np.abs(pb[:, 1:3] - pa[:, 1:3])) < 1e-6 * np.max(np.abs(pa[:, 1:3])), rid
         assert abs(b[rid].value["chi"] / a[rid].value["chi"] - 1.0) < 1e-6, rid
+
+
+def test_table_and_gemm_dst_paths_agree(gpu):
+    """Up to 192 active solves the DST-I passes rebuild the matrix entries from
+    a table in LDS; above that they are a rocBLAS dgemm with the matrix in HBM.
+    The same 100 state points, alone (table passes) and inside a batch of 200
+    (GEMM while more than 192 are active), give the same c(k)."""
+    reqs = serve.chi_requests(200, seed=11, grid=(32, 48, 64), steps=10)
+    pts = [hnc_request(r.p) for r in reqs]
+    nr, rmax, tol, mix, maxiter = pts[0][2:7]
+    a = hnc.hnc_solve_many_device([p[0] for p in pts[:100]], [p[1] for p in pts[:100]], nr, rmax, tol, mix, maxiter)
+    b = hnc.hnc_solve_many_device([p[0] for p in pts], [p[1] for p in pts], nr, rmax, tol, mix, maxiter)
+    assert all(r.converged for r in b)
+    d = max(float(np.max(np.abs(x.ck - y.ck)) / np.max(np.abs(x.ck))) for x, y in zip(a, b[:100]))
+    print("max rel c(k) difference", d)
+    # both stop at max|gamma_new - gamma_s| < 1e-9 (summation order differs)
+    assert d < 1e-8
