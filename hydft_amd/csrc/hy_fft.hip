@@ -237,9 +237,9 @@ __device__ inline cpx<T> load_any(const void* p, size_t i, int is64) {
   return cvt<T>(gload_c<float>(p, i));
 }
 
-// Block shape of the pencil kernels: NA contiguous columns (FP32: 256 bytes
-// of one row for columns longer than 64 points, 192 bytes at 64 and 128 bytes
-// for shorter ones; 128 bytes in FP64) times TPC threads per column; columns
+// Block shape of the pencil kernels: NA contiguous columns (FP32: 192 bytes
+// of one row for columns of 64 and 128 points, 128 bytes for columns of at
+// most 48, 256 bytes otherwise; 128 bytes in FP64) times TPC threads per column; columns
 // of at most 16 points take one thread each, 64 per block.
 template <class T, int L>
 struct PencilShape {
@@ -248,9 +248,11 @@ struct PencilShape {
   // FP32 columns of <= 48 points take 16 per block: small grids keep few x
   // columns (11 at 32^3), and a narrower tile idles fewer lanes (measured
   // 8 % faster batched steps at 32^3 and 48^3; slower at 64^3 and 256^3).
-  // 64-point columns take 24: the 22 kept x columns of a 64^3 grid fill one
-  // tile (2 % faster batched 64^3 steps than 32; 8 and 16 were slower).
-  static constexpr int NA = TPC == 1 ? 64 : (sizeof(T) == 4 ? (L <= 48 ? 16 : (L == 64 ? 24 : 32)) : 8);
+  // 64- and 128-point columns take 24: the 22 kept x columns of a 64^3 grid
+  // fill one tile (2 % faster batched 64^3 steps than 32; 8 and 16 were
+  // slower), the 43 of a 128^3 grid two (3.5 % faster 128^3 Maxwell steps).
+  static constexpr int NA =
+      TPC == 1 ? 64 : (sizeof(T) == 4 ? (L <= 48 ? 16 : (L == 64 || L == 128 ? 24 : 32)) : 8);
   static constexpr int NT = NA * TPC;
   static constexpr int J1 = L2 / TPC;  // step-1 DFTs (length L1) per thread
   static constexpr int J2 = L1 / TPC;  // step-2 DFTs (length L2) per thread
