@@ -1063,8 +1063,21 @@ struct Model {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int g_nsteps = 0, g_probe = 0, g_ring = 0;
+  // probe ring of captured and batched steps: its address is baked into the
+  // graph and into every batch built on the model, so it only grows while
+  // neither holds it (a grown ring drops the graph; a batch refuses)
   double* probe = nullptr;
   int probe_cap = 0;
+  int nbatch = 0;       // live batches that hold this model's arguments
+  // probe rows of hy_model_advance: a buffer of its own, so that an advance
+  // never moves the ring a graph or a batch writes into
+  double* aprobe = nullptr;
+  int aprobe_cap = 0;
+  // work buffers of the real-space diagnostics (mu, budgets), kept
+  struct Work { void* p; size_t bytes; };
+  std::vector<Work> work;
+  double* full_kh = nullptr;  // hartree v(k) on the full half spectrum
+  double* full_kr = nullptr;  // ry -c_cor(k)/beta on the full half spectrum
   int nxslots = 0, nzslots = 0, nfslots = 0;
   size_t xsz = 0, zsz = 0, nt = 0;
   size_t esz = 16;      // compute element size
@@ -1632,9 +1645,15 @@ static double fun_kernel(const Model& m, double k) {
 
 static void model_free(Model& m) {
   for (void* p : {(void*)m.S, (void*)m.R, (void*)m.uold, (void*)m.kcomb, (void*)m.fext, (void*)m.fdtab,
-                  m.xb, m.zb, m.fb, (void*)m.red, (void*)m.ctl, (void*)m.probe, (void*)m.d_xargs,
+                  m.xb, m.zb, m.fb, (void*)m.red, (void*)m.ctl, (void*)m.probe, (void*)m.aprobe, (void*)m.d_xargs,
                   (void*)m.d_real, (void*)m.d_full})
     if (p) (void)hipFree(p);
+  for (auto& w : m.work)
+    if (w.p) (void)hipFree(w.p);
+  m.work.clear();
+  if (m.full_kh) (void)hipFree(m.full_kh);
+  if (m.full_kr) (void)hipFree(m.full_kr);
+  m.full_kh = m.full_kr = nullptr;
   if (m.gexec) (void)hipGraphExecDestroy(m.gexec);
   if (m.graph) (void)hipGraphDestroy(m.graph);
   m.gexec = nullptr;
@@ -1819,14 +1838,45 @@ static int full_kernels(Model& m, double** kh, double** kr) {
   return HY_OK;
 }
 
+// the same tables, built on first use and kept with the model
+static int full_kernels_kept(Model& m, double** kh, double** kr) {
+  if (!m.full_kh) {
+    double *a = nullptr, *b = nullptr;
+    const int rc = full_kernels(m, &a, &b);
+    if (rc != HY_OK) {
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+      return rc;
+    }
+    m.full_kh = a;
+    m.full_kr = b;
+  }
+  *kh = m.full_kh;
+  *kr = m.full_kr;
+  return HY_OK;
+}
+
+// Work buffers of hy_model_mu / hy_model_budgets: the i-th request of a call
+// gets the model's i-th kept buffer, allocated on first use (or when a call
+// needs it larger) and kept until the model is destroyed.  Freeing device
+// memory waits for all queued work on the device, so per-call buffers would
+// stall every other stream each time a run takes its diagnostics.
 struct DevBufs {
-  std::vector<void*> p;
-  ~DevBufs() { for (void* q : p) if (q) (void)hipFree(q); }
+  Model& m;
+  size_t next = 0;
+  explicit DevBufs(Model& mm) : m(mm) {}
   template <class T> int get(T** out, size_t n) {
-    void* q = nullptr;
-    HY_TRY(dmalloc(&q, sizeof(T) * std::max<size_t>(n, 1)));
-    p.push_back(q);
-    *out = (T*)q;
+    const size_t bytes = sizeof(T) * std::max<size_t>(n, 1);
+    if (next == m.work.size()) m.work.push_back({nullptr, 0});
+    auto& w = m.work[next++];
+    if (w.bytes < bytes) {
+      if (w.p) (void)hipFree(w.p);
+      w.p = nullptr;
+      w.bytes = 0;
+      HY_TRY(dmalloc(&w.p, bytes));
+      w.bytes = bytes;
+    }
+    *out = (T*)w.p;
     return HY_OK;
   }
 };
@@ -1841,9 +1891,7 @@ static int real_terms(Model& m, DevBufs& B, const double* dn, bool split, c64** 
   HY_TRY(B.get(&N, nf));
   HY_TRY(B.get(&W, nf));
   double *kh, *kr;
-  HY_TRY(full_kernels(m, &kh, &kr));
-  B.p.push_back(kh);
-  B.p.push_back(kr);
+  HY_TRY(full_kernels_kept(m, &kh, &kr));
   HY_TRY(fft_r2c_full_api(g, dn, N, m.st));
   FullArgs fa = full_args_m(m);
   const int nb = nblocks(nf);
@@ -2217,10 +2265,12 @@ int hy_model_advance(hy_model h, int nsteps, int dt_every, double tend, const in
       int ty = sy >= 0 ? sy : sy + g.Ky, tz = sz >= 0 ? sz : sz + g.Kz;
       pidx = ((long long)tz * g.Ky + ty) * g.Kx + mx;
     }
-    if (m.probe_cap < nsteps) {
-      if (m.probe) (void)hipFree(m.probe);
-      HY_TRY(dmalloc((void**)&m.probe, sizeof(double) * 6 * std::max(nsteps, 1)));
-      m.probe_cap = nsteps;
+    if (m.aprobe_cap < nsteps) {
+      if (m.aprobe) (void)hipFree(m.aprobe);
+      m.aprobe = nullptr;
+      m.aprobe_cap = 0;
+      HY_TRY(dmalloc((void**)&m.aprobe, sizeof(double) * 6 * std::max(nsteps, 1)));
+      m.aprobe_cap = nsteps;
     }
   }
   // tend > 0: stop at tend and clamp the last step (src/hydft.f90:41,44);
@@ -2247,14 +2297,14 @@ int hy_model_advance(hy_model h, int nsteps, int dt_every, double tend, const in
       m.irecompute = (m.irecompute + 1) % dt_every;
     }
     HY_TRY(step_once(m, tclamp));
-    if (probe3) hipLaunchKernelGGL(k_probe, dim3(1), dim3(1), 0, m.st, m.ctl, m.S, m.nt, pidx, m.n0, m.probe + 6 * (size_t)taken);
+    if (probe3) hipLaunchKernelGGL(k_probe, dim3(1), dim3(1), 0, m.st, m.ctl, m.S, m.nt, pidx, m.n0, m.aprobe + 6 * (size_t)taken);
     ++taken;
   }
   HY_TRY(finish_relax(m));
   Ctl c;
   HY_TRY(read_ctl(m, c));
   if (probe3 && taken > 0) {
-    HY_HIP(hipMemcpy(h_probe, m.probe, sizeof(double) * 6 * taken, hipMemcpyDeviceToHost));
+    HY_HIP(hipMemcpy(h_probe, m.aprobe, sizeof(double) * 6 * taken, hipMemcpyDeviceToHost));
   }
   if (h_taken) *h_taken = taken;
   if (c.nonpos) return set_error(HY_ERR_NONPOSITIVE, "density became non-positive; reduce the amplitude or the time step");
@@ -2286,7 +2336,15 @@ static int prepare_steps(Model& m, int nsteps, int dt_every, const int* probe3, 
   if (dt_every > 0 && nsteps % dt_every != 0)
     return set_error(HY_ERR_INVALID, "nsteps must be a multiple of dt_every");
   if (probe3 && m.probe_cap < ring) {
+    if (m.nbatch > 0)
+      return set_error(HY_ERR_INVALID, "the model's probe ring is held by a batch; destroy the batch or ask for "
+                                       "a ring no longer than " + std::to_string(m.probe_cap));
     HY_HIP(hipStreamSynchronize(m.st));
+    // the captured graph writes into the old ring: it goes with it
+    if (m.gexec) { (void)hipGraphExecDestroy(m.gexec); m.gexec = nullptr; }
+    if (m.graph) { (void)hipGraphDestroy(m.graph); m.graph = nullptr; }
+    m.g_probe = 0;
+    m.g_ring = 0;
     if (m.probe) (void)hipFree(m.probe);
     m.probe = nullptr;
     m.probe_cap = 0;
@@ -2334,6 +2392,14 @@ int hy_model_graph_build(hy_model h, int nsteps, int dt_every, const int* probe3
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(m.st, &graph);
   m.prof = prof;
+  // The argument-block uploads made during the capture are copy nodes of the
+  // graph: the device slots hold them only once the graph has run.  Upload
+  // the staging now, so that eager steps before the first replay read the
+  // same blocks (the staging is what the graph copies as well).
+  if (ec == hipSuccess) {
+    HY_HIP(hipMemcpyAsync(m.d_xargs, m.p_xargs, sizeof(PhysArgs) * XA_N, hipMemcpyHostToDevice, m.st));
+    HY_HIP(hipStreamSynchronize(m.st));
+  }
   if (rc != HY_OK) {
     if (graph) (void)hipGraphDestroy(graph);
     return rc;
@@ -2390,10 +2456,12 @@ int hy_model_device_bytes(hy_model h, unsigned long long* bytes) {
   b += sizeof(double) * ((double)m.nt + std::max<double>((double)g.ny * g.nz * 4, 4096));  // kcomb, reductions
   if (m.fext) b += sizeof(double) * 3.0 * g.nreal();
   if (m.fdtab) b += sizeof(double) * (double)fd::NTAB * fd::NC;
-  b += sizeof(double) * 6.0 * m.probe_cap + sizeof(PhysArgs) * XA_N + sizeof(Ctl);
+  b += sizeof(double) * 6.0 * (m.probe_cap + m.aprobe_cap) + sizeof(PhysArgs) * XA_N + sizeof(Ctl);
   b += (double)g.scratch_bytes;
   if (m.d_real) b += sizeof(double) * (double)g.nreal();
   if (m.d_full) b += sizeof(c64) * (double)g.nfull();
+  for (const auto& w : m.work) b += (double)w.bytes;
+  if (m.full_kh) b += 2.0 * sizeof(double) * (double)g.nfull();
   *bytes = (unsigned long long)b;
   return HY_OK;
 }
@@ -2411,11 +2479,14 @@ struct hy_batch_s {
   };
   std::vector<Item> items;
   void* d_all = nullptr;
+  bool holds = false;  // the models count this batch in their nbatch
 };
 
 static void batch_free(hy_batch_s* b) {
   if (!b) return;
   if (b->st) (void)hipStreamSynchronize(b->st);
+  if (b->holds)
+    for (hy_model h : b->models) --h->m.nbatch;
   if (b->d_all) (void)hipFree(b->d_all);
   if (b->st) (void)hipStreamDestroy(b->st);
   delete b;
@@ -2514,6 +2585,8 @@ int hy_batch_create(const hy_model* ms, int n, int nsteps, int dt_every, const i
     b->items.push_back({it.fn_batched, dim3(it.grid.x, it.grid.y, (unsigned)n), it.block,
                         (char*)b->d_all + off[p], it.xline, it.xtw});
   }
+  for (hy_model h : b->models) ++h->m.nbatch;
+  b->holds = true;
   *out = b;
   return HY_OK;
 }
@@ -2618,7 +2691,7 @@ int hy_model_mu(hy_model h, const double* h_n, double* h_mu) {
   if (!h || !h_n || !h_mu) return set_error(HY_ERR_INVALID, "null argument");
   Model& m = h->m;
   const size_t nr = m.g.nreal();
-  DevBufs B;
+  DevBufs B(m);
   double *dn, *out;
   int* flag;
   HY_TRY(B.get(&dn, nr));
@@ -2647,7 +2720,7 @@ int hy_model_budgets(hy_model h, double* out6) {
   const Grid& g = m.g;
   HY_TRY(finish_relax(m));
   const size_t nr = g.nreal();
-  DevBufs B;
+  DevBufs B(m);
   double *dn, *dmom, *dn0;
   c64* F;
   int* flag;

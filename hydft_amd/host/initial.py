@@ -93,19 +93,24 @@ def _kept_half_spectrum(p):
 
 
 def _hermitian_field(spec, amp_k, rng, mask):
-    """Real field whose half spectrum has modulus amp_k and uniform random phases.
-    The kx = 0 (and Nyquist) plane is made Hermitian by the backward transform,
-    which keeps only the real part of the implied full spectrum."""
+    """Real field whose half spectrum has modulus amp_k (masked, zero mean)
+    and uniform random phases.  On the kx = 0 plane a mode and its partner
+    (0, -ky, -kz) are one Fourier pair of a real field: the phase is drawn for
+    the first of the pair (kz > 0, or kz = 0 and ky >= 0, with signed wrapped
+    indices) and the partner takes its negative, so the modulus is amp_k
+    there too.  Self-partnered modes other than k = 0 lie on Nyquist planes,
+    which the mask removes; kx Nyquist is masked likewise."""
     ph = rng.random(amp_k.shape) * TWOPI
-    c = amp_k * np.exp(1j * ph) * mask
-    c[0, 0, 0] = 0.0
-    # symmetrise the kx = 0 plane: c(0, -ky, -kz) = conj c(0, ky, kz)
-    p0 = c[:, :, 0]
-    ny, nz = p0.shape[1], p0.shape[0]
+    nz, ny = amp_k.shape[0], amp_k.shape[1]
+    wy = np.where(np.arange(ny) <= ny // 2, np.arange(ny), np.arange(ny) - ny)
+    wz = np.where(np.arange(nz) <= nz // 2, np.arange(nz), np.arange(nz) - nz)
+    first = (wz[:, None] > 0) | ((wz[:, None] == 0) & (wy[None, :] >= 0))
     iy = (-np.arange(ny)) % ny
     iz = (-np.arange(nz)) % nz
-    sym = 0.5 * (p0 + np.conj(p0[np.ix_(iz, iy)]))
-    c[:, :, 0] = sym
+    p0 = ph[:, :, 0]
+    ph[:, :, 0] = np.where(first, p0, -p0[np.ix_(iz, iy)])
+    c = amp_k * np.exp(1j * ph) * mask
+    c[0, 0, 0] = 0.0
     return spec.bwd(c)
 
 

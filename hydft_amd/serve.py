@@ -348,6 +348,60 @@ def summary(results, wall: float) -> dict:
     return out
 
 
+def stream_requests(n: int, seed: int = 5, grids=(32, 64, 96, 128), weights=(0.4, 0.3, 0.2, 0.1),
+                    steps=(500, 5000), arrival_rate=None) -> list:
+    """Requests of the serving stream: grid drawn from `grids` with `weights`;
+    species ions (Gamma log-uniform in [1, 150], kappa uniform in [0.5, 3],
+    constant eta = 0.1) or electrons (r_s log-uniform in [0.5, 4], theta
+    log-uniform in [0.25, 4], electron_fit viscosity, terms
+    tf,gradient,hartree,ry), 50/50; closure Newtonian or Maxwell (tau = 1),
+    50/50; kind "chi" (drive 1e-4 cos(q x - w t), w uniform in [0.1, 3]) or
+    "dispersion" (single mode of amplitude 1e-4), 50/50; q uniform in [0.2, 2]
+    (box 2 pi/q); step count uniform in `steps`, a multiple of 20; CFL time
+    step every 20 steps.  `arrival_rate`: Poisson arrivals at that rate (1/s),
+    drawn from their own stream (the same runs with or without them)."""
+    rng = np.random.default_rng(seed)
+    arr = np.random.default_rng([seed, 1])
+    cw = np.cumsum(np.asarray(weights, float) / float(np.sum(weights)))
+    lo, hi = int(steps[0]) // DT_EVERY, int(steps[1]) // DT_EVERY
+    out = []
+    t = 0.0
+    for i in range(n):
+        N = int(grids[int(np.searchsorted(cw, rng.random(), side="right"))])
+        electron = rng.random() < 0.5
+        maxwell = rng.random() < 0.5
+        chi = rng.random() < 0.5
+        q = float(rng.uniform(0.2, 2.0))
+        p = Params()
+        if electron:
+            p.kind = "electron"
+            p.rs = float(np.exp(rng.uniform(np.log(0.5), np.log(4.0))))
+            p.theta = float(np.exp(rng.uniform(np.log(0.25), np.log(4.0))))
+            p.transport = "electron_fit"
+        else:
+            p.kind = "ion"
+            p.gamma = float(np.exp(rng.uniform(0.0, np.log(150.0))))
+            p.kappa = float(rng.uniform(0.5, 3.0))
+            p.eta = 0.1
+        p.nx = p.ny = p.nz = N
+        p.lx = p.ly = p.lz = TWOPI / q
+        if maxwell:
+            p.closure, p.tau_model, p.tau = "maxwell", "constant", 1.0
+        w = float(rng.uniform(0.1, 3.0))
+        nsteps = DT_EVERY * int(rng.integers(lo, hi + 1))
+        if chi:
+            p.vext, p.vext_amplitude, p.vext_mode = "driven", 1.0e-4, (1, 0, 0)
+            p.vext_omega, p.vext_ramp = w, 1.0
+        else:
+            p.ic, p.ic_amplitude, p.ic_mode = "mode", 1.0e-4, (1, 0, 0)
+        p.dt, p.cfl = 0.0, 0.4
+        p.finalize()
+        if arrival_rate:
+            t += float(arr.exponential(1.0 / arrival_rate))
+        out.append(Request(i, p, nsteps, "chi" if chi else "dispersion", (1, 0, 0), t))
+    return out
+
+
 def chi_requests(n: int, seed: int = 7, grid=(32,), steps=400, arrival_rate=None, kind="ion") -> list:
     """Driven-mode scan requests in the spirit of the batched configuration:
     Yukawa ions, Gamma log-uniform in [1, 100], kappa uniform in [0.5, 3],

@@ -40,6 +40,20 @@ def oracle_model(p: Params, st):
                         vext_omega=p.vext_omega, vext_width=p.vext_width, vext_ramp=p.vext_ramp)
 
 
+def oracle_from_species(p: Params, sp):
+    """The oracle built from a resolved species (hydft_amd.model.build_species):
+    the same c(k) table, the transport scalars of its fit (eta_bar, xi_bar,
+    tau, eta_l) and the same time-step settings."""
+    st = sp.structure
+    return O.make_model(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, kind=p.kind, gamma=p.gamma, kappa=p.kappa,
+                        rs=p.rs, mass=p.mass, terms=p.terms,
+                        k_tab=None if st is None else st.k, c_tab=None if st is None else st.c,
+                        tfk_gamma=p.tfk_gamma, closure=sp.closure, eta=sp.transport.eta, xi=sp.transport.xi,
+                        tau=sp.tau, eta_l=sp.transport.eta_l, dt=p.dt, cfl=p.cfl, dealias=p.dealias,
+                        vext=p.vext, vext_amplitude=p.vext_amplitude, vext_mode=p.vext_mode,
+                        vext_omega=p.vext_omega, vext_width=p.vext_width, vext_ramp=p.vext_ramp)
+
+
 def smooth_fields(p: Params, seed, amp_n=0.02, amp_u=0.0, with_pi=False):
     """Band-limited random n, mom (and Pi): seeded uniform deviates, dealiased."""
     spc = O.Spec(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, p.dealias)

@@ -1,8 +1,8 @@
 """GPU model parity against the CPU oracle on identical inputs (FP64).
 
 For each case: model_rhs at t = 0 (relative max error <= 1e-10 of the field
-scale) and the state after a few RK3 steps (relative L2 of n, mom, Pi <= 1e-11,
-and of n - n0 <= 1e-9).  Cases cover ions / electrons, Newtonian with and
+scale) and the state after a few RK3 steps (relative L2 of n, n - n0, every
+momentum component and every stress component <= 1e-11).  Cases cover ions / electrons, Newtonian with and
 without viscosity, Maxwell, 1-D / 2-D / 3-D, and the three external
 potentials."""
 import zlib
@@ -97,14 +97,16 @@ def test_steps_match_oracle(gpu, name):
     assert taken == steps
     n, mom, pi = gm.get_state()
     n0 = P.N0
-    assert P.rel_l2(n, om.n) <= 1e-11, name
-    assert P.rel_l2(n - n0, om.n - n0) <= 1e-9, name
+    e = {"n": P.rel_l2(n, om.n), "dn": P.rel_l2(n - n0, om.n - n0)}
     for d in range(p.ny > 1 and (3 if p.nz > 1 else 2) or 1):
-        assert P.rel_l2(mom[d], om.mom[d]) <= 1e-9, (name, d)
+        e[f"mom{d}"] = P.rel_l2(mom[d], om.mom[d])
     if om.stress.evolve:
         for c in range(6):
             if np.linalg.norm(om.pi[c]) > 0:
-                assert P.rel_l2(pi[c], om.pi[c]) <= 1e-9, (name, c)
+                e[f"pi{c}"] = P.rel_l2(pi[c], om.pi[c])
+    print(name, " ".join(f"{k}={v:.2e}" for k, v in e.items()))
+    for k, v in e.items():
+        assert v <= 1e-11, (name, k, v)
     t, dt, step = gm.time
     assert step == steps and abs(t - om.t) <= 1e-14 * max(1.0, om.t)
 

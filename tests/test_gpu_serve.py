@@ -114,3 +114,46 @@ def test_served_chi_carries_linear_theory(gpu):
     out = serve.Server(precision="mixed", batch=True).serve(reqs, realtime=False)
     for r in out:
         assert "chi_linear" in r.value and np.isfinite(r.value["chi_rel_dev"])
+
+
+@pytest.mark.gpu
+def test_advance_keeps_the_graph_ring(gpu):
+    """hy_model_advance with more probed steps than a captured graph's ring
+    leaves the ring where the graph writes: replays after it record the same
+    rows as eager steps of a twin model."""
+    req = _req(0, N=16)
+    cache = serve._hnc.StructureCache()
+    a = serve.make_model(req, "fp64", cache)
+    b = serve.make_model(req, "fp64", cache)
+    a.graph_build(20, dt_every=20, probe=req.mode, ring=20)
+    ta, rows_a = a.advance(40, dt_every=20, probe=req.mode)
+    tb, rows_b0 = b.advance(40, dt_every=20, probe=req.mode)
+    assert ta == tb == 40 and np.array_equal(rows_a, rows_b0)
+    a.graph_launch(1)
+    rows_g = a.graph_probe()
+    _, rows_b = b.advance(20, dt_every=20, probe=req.mode)
+    assert np.array_equal(rows_g, rows_b)
+    assert np.array_equal(a.get_state()[0], b.get_state()[0])
+    a.close(); b.close()
+
+
+@pytest.mark.gpu
+def test_batch_holds_the_probe_ring(gpu):
+    """While a batch holds a model, the model's probe ring cannot grow (the
+    batch writes into it); after the batch is destroyed it can."""
+    from hydft_amd import abi
+    from hydft_amd.model import Batch
+    reqs = [_req(i, N=16) for i in range(2)]
+    cache = serve._hnc.StructureCache()
+    ms = [serve.make_model(r, "fp64", cache) for r in reqs]
+    bt = Batch(ms, 20, dt_every=20, probe=(1, 0, 0), ring=20)
+    with pytest.raises(abi.HydftError) as e:
+        ms[0].graph_build(20, dt_every=20, probe=(1, 0, 0), ring=40)
+    assert e.value.code == abi.HY_ERR_INVALID
+    ms[0].graph_build(20, dt_every=20, probe=(1, 0, 0), ring=20)  # same ring: allowed
+    bt.close()
+    ms[0].graph_build(20, dt_every=20, probe=(1, 0, 0), ring=40)
+    ms[0].graph_launch(1)
+    assert np.all(np.isfinite(ms[0].graph_probe()[:20]))
+    for m in ms:
+        m.close()
