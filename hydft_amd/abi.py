@@ -88,6 +88,7 @@ _SIGS = {
     "hy_spec_convolve": [_P, _P, _P, _P, _P],
     "hy_spec_dealias": [_P, _P, _P],
     "hy_spec_mean": [_P, _P, _PD, _P],
+    "hy_spec_mean_noalloc": [_P, _P, _PD, _P],
     "hy_spec_mode": [_P, _P, _I, _I, _I, _PD, _P],
     "hy_stress_pi0": [_P, _D, _D, _P, _P, _P],
     "hy_stress_div": [_P, _P, _P, _P],

@@ -895,6 +895,72 @@ int hy_spec_mean(hy_grid gh, const double* d_u, double* h_mean, void* stream) {
   return HY_OK;
 }
 
+}  // extern "C"
+
+// ---- spec_mean without device allocation
+// The partial sums of hy_spec_mean_noalloc live in this file-scope device
+// array (one copy per GPU, held until the process ends): MEAN_PARTS partials
+// and the total.  Host code never passes the array's name to a kernel (in
+// host code it is a stand-in, not a device address); it passes the address
+// that hipGetSymbolAddress returns for the current device.
+namespace hy {
+constexpr int MEAN_PARTS = 512;
+constexpr int MEAN_SCRATCH = 1024;  // doubles (8 KiB)
+static_assert(MEAN_PARTS + 1 <= MEAN_SCRATCH, "mean scratch too small");
+}  // namespace hy
+static __device__ __align__(256) double g_mean_scratch[hy::MEAN_SCRATCH];
+
+namespace hy {
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_mean_mutex;
+static double* g_mean_ptr[MAX_DEVICES] = {nullptr};
+
+// Device address of g_mean_scratch on the current device, looked up on the
+// first call on that device.
+static int mean_scratch_ptr(double** out) {
+  int dev = -1;
+  HY_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return set_error(HY_ERR_INVALID, "spec_mean_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_mean_mutex);
+  if (!g_mean_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(g_mean_scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return set_error(HY_ERR_HIP, std::string("spec_mean_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    }
+    g_mean_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_mean_ptr[dev];
+  return HY_OK;
+}
+}  // namespace hy
+
+extern "C" {
+
+int hy_spec_mean_noalloc(hy_grid gh, const double* d_u, double* h_mean, void* stream) {
+  if (!gh || !d_u || !h_mean) return set_error(HY_ERR_INVALID, "null argument");
+  const Grid& g = gh->g;
+  const size_t n = g.nreal();
+  // the partials do not grow with the field: any grid the library accepts
+  // fits; the check guards the fixed layout of the array
+  if (n == 0 || MEAN_PARTS + 1 > MEAN_SCRATCH)
+    return set_error(HY_ERR_INVALID, "spec_mean_noalloc: field size outside the scratch layout (" +
+                                         std::to_string(MEAN_SCRATCH) + " doubles)");
+  double* red = nullptr;
+  HY_TRY(mean_scratch_ptr(&red));
+  hipStream_t st = (hipStream_t)stream;
+  // the same kernels and partial count as hy_spec_mean: the same sum, bit for bit
+  hipLaunchKernelGGL(k_sum_partial, dim3(MEAN_PARTS), dim3(256), 0, st, d_u, n, red + 1);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, st, red + 1, MEAN_PARTS, red);
+  HY_HIP(hipGetLastError());
+  double sum = 0.0;
+  HY_HIP(hipMemcpyAsync(&sum, red, sizeof(double), hipMemcpyDeviceToHost, st));
+  HY_HIP(hipStreamSynchronize(st));
+  *h_mean = sum / (double)n;
+  return HY_OK;
+}
+
 int hy_spec_mode(hy_grid gh, const double* d_u, int mx, int my, int mz, double* h_re_im,
                  void* stream) {
   if (!gh || !d_u || !h_re_im) return set_error(HY_ERR_INVALID, "null argument");

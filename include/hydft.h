@@ -78,6 +78,15 @@ int hy_spec_dealias(hy_grid g, double* d_u, void* stream);
 int hy_spec_mean(hy_grid g, const double* d_u, double* h_mean, void* stream);
 int hy_spec_mode(hy_grid g, const double* d_u, int mx, int my, int mz, double* h_re_im,
                  void* stream);
+/* hy_spec_mean without device allocation.  No counterpart in the reference:
+   for callers that cannot allocate device memory (tight serving loops,
+   harnesses that forbid hipMalloc).  hy_spec_mean takes its partial sums from
+   the grid's operator scratch (eight half spectra, allocated on a grid's first
+   operator call); this variant keeps them in a fixed 8 KiB device array per
+   GPU (513 doubles used, for every grid size) and calls no allocator.  Same
+   kernels, same result bit for bit.  Calls that overlap on one GPU share the
+   array and race: serialise them. */
+int hy_spec_mean_noalloc(hy_grid g, const double* d_u, double* h_mean, void* stream);
 
 /* ---- fermi_dirac.f90 functions (host): which = 0 I_{-1/2}, 1 I_{1/2},
    2 I_{3/2}, 3 I'_{-1/2}, 4 I''_{-1/2}, 5 xi, 6 dxi/dalpha, 7 inverse of I_{1/2}
