@@ -126,6 +126,13 @@ _SIGS = {
     "hy_batch_destroy": [_P],
     "hy_model_probe_ring": [_P, _I, _P],
     "hy_model_kept": [_P, _PI],
+    "hy_fun_create": [_P, _I, _P, C.POINTER(_P)],
+    "hy_fun_destroy": [_P],
+    "hy_fun_nterms": [_P, _PI],
+    "hy_fun_add_mu": [_P, _I, _P, _P, _P, _P],
+    "hy_fun_mu": [_P, _P, _P, _P, _P],
+    "hy_fun_kernel": [_P, _I, _D, _PD],
+    "hy_fun_energy": [_P, _I, _P, _P, _PD, _P],
     "hy_hnc_iterate": [_I, _I, C.c_double, C.c_double, _P, _P, _P, _P, C.c_double, C.c_double, _I, _P, _P, _P, _P],
 }
 

@@ -715,9 +715,6 @@ int spec_grad_api(Grid& g, const double* u, double* du3, hipStream_t s) { return
 // ======================================================================= ABI
 using namespace hy;
 
-struct hy_grid_s {
-  Grid g;
-};
 
 extern "C" {
 

@@ -287,3 +287,8 @@ int pencil_pass(const Grid& g, int axis, bool inverse, bool trunc, int A, int B,
                 const PencilField* f, int nf, hipStream_t s);
 
 }  // namespace hy
+
+// The grid handle of the C ABI (hy_grid): a grid and its transform plans.
+struct hy_grid_s {
+  hy::Grid g;
+};

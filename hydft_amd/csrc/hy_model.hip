@@ -1613,9 +1613,10 @@ static double interp_linear(const std::vector<double>& x, const std::vector<doub
   return (1.0 - w) * y[lo] + w * y[lo + 1];
 }
 
-// Per-term linear kernels K_i(k) (free_energy.f90:22-59 and each term's kernel)
-static double term_kernel(const Model& m, size_t t, double k) {
-  const hy_term_desc& d = m.terms[t];
+// Per-term linear kernels K_i(k) (free_energy.f90:22-59 and each term's kernel);
+// kt, ct: the RY term's radial table
+static double term_kernel_desc(const hy_term_desc& d, const std::vector<double>& kt, const std::vector<double>& ct,
+                               double k) {
   const double* tab = fd::host_table();
   const double pi = 3.14159265358979323846264338327950288;
   switch (d.kind) {
@@ -1627,8 +1628,6 @@ static double term_kernel(const Model& m, size_t t, double k) {
       return den > 0.0 ? 4.0 * pi * d.e2 / den : 0.0;
     }
     case HY_TERM_RY: {
-      const auto& kt = m.ktabs[t];
-      const auto& ct = m.ctabs[t];
       double k1 = std::max(k, kt[0]);
       double c = interp_linear(kt, ct, std::min(k1, kt.back()));
       if (d.subtract_mean_field) c = c + d.beta * 4.0 * pi * d.e2 / (k1 * k1 + d.kappa * d.kappa);
@@ -1636,6 +1635,9 @@ static double term_kernel(const Model& m, size_t t, double k) {
     }
   }
   return 0.0;
+}
+static double term_kernel(const Model& m, size_t t, double k) {
+  return term_kernel_desc(m.terms[t], m.ktabs[t], m.ctabs[t], k);
 }
 static double fun_kernel(const Model& m, double k) {
   double kk = 0.0;
@@ -2832,6 +2834,326 @@ int hy_fd_eval(int which, int n, const double* x, double* y) {
       default: return set_error(HY_ERR_INVALID, "hy_fd_eval: unknown function");
     }
   }
+  return HY_OK;
+}
+
+}  // extern "C"
+
+// ============================================ functional terms on device buffers
+// functional_t / fe_term_t (functional.f90:17-90, free_energy.f90:22-59) on a
+// drop-in grid: per-term add_mu, kernel and energy of the reference's terms on
+// FP64 device fields in the reference layouts.
+namespace hy {
+
+// W = mask K (N - n0sub delta_k0) on the full half spectrum
+__global__ void k_fun_conv(FullArgs a, const c64* __restrict__ N, const double* __restrict__ kern, double n0sub,
+                           c64* __restrict__ W) {
+  const size_t n = (size_t)a.nkx * a.ny * a.nz;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % a.nkx), j = (int)((e / a.nkx) % a.ny), k = (int)(e / ((size_t)a.nkx * a.ny));
+    c64 v = N[e];
+    if (e == 0) v.x -= n0sub;
+    W[e] = scal(v, fkept(a, i, j, k) ? kern[e] : 0.0);
+  }
+}
+
+__global__ void k_fun_fill(double* __restrict__ u, size_t n, double v) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) u[e] = v;
+}
+__global__ void k_fun_clear(int* flag) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0;
+}
+
+struct FunPointArgs {
+  size_t nr;
+  int kind;              // HY_TERM_*
+  const double* n;
+  const double* w;       // hartree / ry: the convolution field, added as is
+  const double* gn;      // gradient: 3 fields
+  const double* lap;
+  double beta, cn, cg;
+  const double* fdtab;
+  double* mu;
+  int* flag;
+};
+
+// mu += the term's pointwise part (fe_ideal_classical.f90:33-41,
+// fe_kinetic_tfk.f90:63-82 and 114-138, and the convolution terms)
+__global__ void k_fun_point(FunPointArgs a) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < a.nr; e += (size_t)gridDim.x * blockDim.x) {
+    if (a.kind == HY_TERM_HARTREE || a.kind == HY_TERM_RY) {
+      a.mu[e] += a.w[e];
+      continue;
+    }
+    const double n = a.n[e];
+    if (!(n > 0.0)) { *a.flag = 1; continue; }
+    if (a.kind == HY_TERM_IDEAL) {
+      a.mu[e] += log(n) / a.beta;
+    } else if (a.kind == HY_TERM_TF) {
+      a.mu[e] += fd::alpha_from_ip12(a.fdtab, n / a.cn) / a.beta;
+    } else if (a.kind == HY_TERM_GRADIENT) {
+      const double al = fd::alpha_from_ip12(a.fdtab, n / a.cn);
+      const double xi = fd::xi(a.fdtab, al);
+      const double dxidn = fd::dxi(a.fdtab, al) / (0.5 * a.cn * fd::im12(a.fdtab, al));
+      const double g2 = a.gn[e] * a.gn[e] + a.gn[a.nr + e] * a.gn[a.nr + e] + a.gn[2 * a.nr + e] * a.gn[2 * a.nr + e];
+      a.mu[e] -= a.cg * (dxidn * g2 + 2.0 * xi * a.lap[e]);
+    }
+  }
+}
+
+}  // namespace hy
+
+struct hy_fun_s {
+  hy::Grid* g = nullptr;                    // the grid's (borrowed: the grid outlives the functional)
+  std::vector<hy_term_desc> terms;
+  std::vector<std::vector<double>> kt, ct;  // RY radial tables
+  std::vector<double*> ktab;                // hartree / ry: K_i(|k|) on the full half spectrum (device)
+  double* fdtab = nullptr;
+  c64* W = nullptr;      // work spectrum (destroyed by each backward transform)
+  double* w = nullptr;   // convolution field
+  double* gn = nullptr;  // gradient term: grad n (3 fields) and lap n
+  double* lap = nullptr;
+  double* part = nullptr;  // energy partial sums: 256 blocks x 8
+  int* flag = nullptr;
+};
+
+namespace {
+constexpr int FUN_NB = 256;
+
+void fun_free(hy_fun_s* f) {
+  if (!f) return;
+  for (double* p : f->ktab) if (p) (void)hipFree(p);
+  for (void* p : {(void*)f->fdtab, (void*)f->W, (void*)f->w, (void*)f->gn, (void*)f->lap, (void*)f->part,
+                  (void*)f->flag})
+    if (p) (void)hipFree(p);
+  delete f;
+}
+
+int fun_term_index(const hy_fun_s* f, int term) {
+  if (!f || term < 0 || term >= (int)f->terms.size()) return set_error(HY_ERR_INVALID, "hy_fun: no such term");
+  return HY_OK;
+}
+
+// mu += dF_t/dn for term t, on stream st (no synchronisation)
+int fun_add_mu_async(hy_fun_s* f, int t, const double* n, const c64* N, double* mu, hipStream_t st) {
+  Grid& g = *f->g;
+  const hy_term_desc& d = f->terms[t];
+  const size_t nr = g.nreal(), nf = g.nfull();
+  FullArgs fa;
+  fa.nkx = g.nkx; fa.ny = g.ny; fa.nz = g.nz; fa.Kx = g.Kx; fa.My = g.My; fa.Mz = g.Mz;
+  fa.kx = g.d_kx; fa.ky = g.d_ky; fa.kz = g.d_kz;
+  FunPointArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.nr = nr; a.kind = d.kind; a.n = n; a.mu = mu; a.flag = f->flag;
+  a.beta = d.beta; a.cn = d.cn; a.cg = d.cg; a.fdtab = f->fdtab;
+  if (d.kind == HY_TERM_HARTREE || d.kind == HY_TERM_RY) {
+    // hartree: mu += B(v N mask); ry: mu -= B(c_cor (N - n0 delta_k0) mask) / beta
+    hipLaunchKernelGGL(k_fun_conv, dim3(nblocks(nf)), dim3(256), 0, st, fa, N, f->ktab[t],
+                       d.kind == HY_TERM_RY ? d.n0 : 0.0, f->W);
+    HY_TRY(fft_c2r_full_api(g, f->W, f->w, st));
+    a.w = f->w;
+  } else if (d.kind == HY_TERM_GRADIENT) {
+    for (int dd = 0; dd < 3; ++dd) {
+      double* dst = f->gn + nr * dd;
+      if (dd >= g.ndim) {
+        hipLaunchKernelGGL(k_fun_fill, dim3(nblocks(nr)), dim3(256), 0, st, dst, nr, 0.0);
+        continue;
+      }
+      const double* kdir = dd == 0 ? g.d_kx : (dd == 1 ? g.d_ky : g.d_kz);
+      const size_t kstride = dd == 0 ? 1 : (dd == 1 ? (size_t)g.nkx : (size_t)g.nkx * g.ny);
+      const int kn = dd == 0 ? g.nkx : (dd == 1 ? g.ny : g.nz);
+      hipLaunchKernelGGL(k_full_conv, dim3(nblocks(nf)), dim3(256), 0, st, fa, 1, kdir, kstride, kn, N,
+                         (const double*)nullptr, (const double*)nullptr, 0.0, f->W);
+      HY_TRY(fft_c2r_full_api(g, f->W, dst, st));
+    }
+    hipLaunchKernelGGL(k_full_conv, dim3(nblocks(nf)), dim3(256), 0, st, fa, 2, g.d_kx, (size_t)1, g.nkx, N,
+                       (const double*)nullptr, (const double*)nullptr, 0.0, f->W);
+    HY_TRY(fft_c2r_full_api(g, f->W, f->lap, st));
+    a.gn = f->gn;
+    a.lap = f->lap;
+  }
+  hipLaunchKernelGGL(k_fun_point, dim3(nblocks(nr)), dim3(256), 0, st, a);
+  HY_HIP(hipGetLastError());
+  return HY_OK;
+}
+
+int fun_check(hy_fun_s* f, hipStream_t st) {
+  int bad = 0;
+  HY_HIP(hipMemcpyAsync(&bad, f->flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  HY_HIP(hipStreamSynchronize(st));
+  if (bad) return set_error(HY_ERR_NONPOSITIVE, "functional: density is not positive");
+  return HY_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int hy_fun_create(hy_grid gh, int nterms, const hy_term_desc* terms, hy_fun* out) {
+  if (!gh || !out || nterms < 0 || (nterms > 0 && !terms)) return set_error(HY_ERR_INVALID, "hy_fun_create: bad argument");
+  auto* f = new hy_fun_s();
+  f->g = &gh->g;
+  Grid& g = gh->g;
+  f->terms.assign(terms, terms + nterms);
+  f->kt.resize(nterms);
+  f->ct.resize(nterms);
+  f->ktab.assign(nterms, nullptr);
+  bool grad = false, fd_needed = false;
+  auto fail = [&](int rc) { fun_free(f); return rc; };
+  for (int t = 0; t < nterms; ++t) {
+    const hy_term_desc& d = terms[t];
+    if (d.kind < HY_TERM_IDEAL || d.kind > HY_TERM_RY) return fail(set_error(HY_ERR_UNKNOWN_KIND, "unknown free-energy term kind"));
+    if (!(d.beta > 0.0) || !(d.n0 > 0.0)) return fail(set_error(HY_ERR_INVALID, "term: beta and n0 must be > 0"));
+    if ((d.kind == HY_TERM_TF || d.kind == HY_TERM_GRADIENT) && !(d.cn > 0.0))
+      return fail(set_error(HY_ERR_INVALID, "tf/gradient term needs cn > 0 (kind = electron)"));
+    if (d.kind == HY_TERM_RY) {
+      if (d.ntab < 1 || !d.k_tab || !d.c_tab) return fail(set_error(HY_ERR_INVALID, "ry term needs a structure table"));
+      f->kt[t].assign(d.k_tab, d.k_tab + d.ntab);
+      f->ct[t].assign(d.c_tab, d.c_tab + d.ntab);
+    }
+    grad |= d.kind == HY_TERM_GRADIENT;
+    fd_needed |= d.kind == HY_TERM_TF || d.kind == HY_TERM_GRADIENT;
+  }
+  const size_t nr = g.nreal(), nf = g.nfull();
+  // convolution tables: the terms' own kernels K_i(|k|) (fe_hartree.f90:41-43,
+  // fe_correlation_ry.f90:59-61 with the 1/beta of ry_add_mu folded in)
+  for (int t = 0; t < nterms; ++t) {
+    const hy_term_desc& d = f->terms[t];
+    if (d.kind != HY_TERM_HARTREE && d.kind != HY_TERM_RY) continue;
+    std::vector<double> h(nf);
+    for (int k = 0; k < g.nz; ++k)
+      for (int j = 0; j < g.ny; ++j)
+        for (int i = 0; i < g.nkx; ++i) {
+          const double kabs = std::sqrt(g.kx[i] * g.kx[i] + g.ky[j] * g.ky[j] + g.kz[k] * g.kz[k]);
+          h[((size_t)k * g.ny + j) * g.nkx + i] = term_kernel_desc(d, f->kt[t], f->ct[t], kabs);
+        }
+    if (dmalloc((void**)&f->ktab[t], sizeof(double) * nf) != HY_OK) return fail(HY_ERR_OOM);
+    if (hipMemcpy(f->ktab[t], h.data(), sizeof(double) * nf, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(set_error(HY_ERR_HIP, "hy_fun_create: upload"));
+  }
+  if (fd_needed) {
+    if (dmalloc((void**)&f->fdtab, sizeof(double) * fd::NTAB * fd::NC) != HY_OK) return fail(HY_ERR_OOM);
+    if (hipMemcpy(f->fdtab, fd::host_table(), sizeof(double) * fd::NTAB * fd::NC, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(set_error(HY_ERR_HIP, "hy_fun_create: upload"));
+  }
+  if (dmalloc((void**)&f->W, sizeof(c64) * nf) != HY_OK || dmalloc((void**)&f->w, sizeof(double) * nr) != HY_OK ||
+      dmalloc((void**)&f->part, sizeof(double) * FUN_NB * 8) != HY_OK || dmalloc((void**)&f->flag, sizeof(int)) != HY_OK)
+    return fail(HY_ERR_OOM);
+  if (grad && (dmalloc((void**)&f->gn, sizeof(double) * nr * 3) != HY_OK ||
+               dmalloc((void**)&f->lap, sizeof(double) * nr) != HY_OK))
+    return fail(HY_ERR_OOM);
+  *out = f;
+  return HY_OK;
+}
+
+int hy_fun_destroy(hy_fun f) {
+  fun_free(f);
+  return HY_OK;
+}
+
+int hy_fun_nterms(hy_fun f, int* n) {
+  if (!f || !n) return set_error(HY_ERR_INVALID, "null argument");
+  *n = (int)f->terms.size();
+  return HY_OK;
+}
+
+int hy_fun_add_mu(hy_fun f, int term, const double* d_n, const double* d_nhat, double* d_mu, void* stream) {
+  HY_TRY(fun_term_index(f, term));
+  if (!d_n || !d_nhat || !d_mu) return set_error(HY_ERR_INVALID, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fun_clear, dim3(1), dim3(1), 0, st, f->flag);
+  HY_TRY(fun_add_mu_async(f, term, d_n, reinterpret_cast<const c64*>(d_nhat), d_mu, st));
+  return fun_check(f, st);
+}
+
+int hy_fun_mu(hy_fun f, const double* d_n, const double* d_nhat, double* d_mu, void* stream) {
+  if (!f || !d_n || !d_nhat || !d_mu) return set_error(HY_ERR_INVALID, "null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t nr = f->g->nreal();
+  hipLaunchKernelGGL(k_fun_clear, dim3(1), dim3(1), 0, st, f->flag);
+  hipLaunchKernelGGL(k_fun_fill, dim3(nblocks(nr)), dim3(256), 0, st, d_mu, nr, 0.0);  // functional.f90:61
+  for (int t = 0; t < (int)f->terms.size(); ++t)
+    HY_TRY(fun_add_mu_async(f, t, d_n, reinterpret_cast<const c64*>(d_nhat), d_mu, st));
+  return fun_check(f, st);
+}
+
+int hy_fun_kernel(hy_fun f, int term, double k, double* h_K) {
+  if (!f || !h_K) return set_error(HY_ERR_INVALID, "null argument");
+  if (term >= 0) {
+    HY_TRY(fun_term_index(f, term));
+    *h_K = term_kernel_desc(f->terms[term], f->kt[term], f->ct[term], k);
+    return HY_OK;
+  }
+  double kk = 0.0;  // functional.f90:70-78
+  for (size_t t = 0; t < f->terms.size(); ++t) kk += term_kernel_desc(f->terms[t], f->kt[t], f->ct[t], k);
+  *h_K = kk;
+  return HY_OK;
+}
+
+int hy_fun_energy(hy_fun f, int term, const double* d_n, const double* d_nhat, double* h_F, void* stream) {
+  if (!f || !d_n || !d_nhat || !h_F) return set_error(HY_ERR_INVALID, "null argument");
+  if (term >= 0) HY_TRY(fun_term_index(f, term));
+  hipStream_t st = (hipStream_t)stream;
+  Grid& g = *f->g;
+  const size_t nr = g.nreal(), nf = g.nfull();
+  const c64* N = reinterpret_cast<const c64*>(d_nhat);
+  FullArgs fa;
+  fa.nkx = g.nkx; fa.ny = g.ny; fa.nz = g.nz; fa.Kx = g.Kx; fa.My = g.My; fa.Mz = g.Mz;
+  fa.kx = g.d_kx; fa.ky = g.d_ky; fa.kz = g.d_kz;
+  double total = 0.0;
+  const int t0 = term >= 0 ? term : 0, t1 = term >= 0 ? term + 1 : (int)f->terms.size();
+  for (int t = t0; t < t1; ++t) {
+    const hy_term_desc& d = f->terms[t];
+    PointArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nr = nr; a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nd = 0;  // no momentum sums
+    a.n = d_n; a.n0 = d.n0; a.beta = d.beta; a.cn = d.cn; a.alpha0 = d.alpha0; a.cg = d.cg;
+    a.fdtab = f->fdtab; a.vext_kind = HY_VEXT_NONE;
+    a.out = f->part; a.flag = f->flag;
+    if (d.kind == HY_TERM_HARTREE || d.kind == HY_TERM_RY) {
+      hipLaunchKernelGGL(k_fun_conv, dim3(nblocks(nf)), dim3(256), 0, st, fa, N, f->ktab[t],
+                         d.kind == HY_TERM_RY ? d.n0 : 0.0, f->W);
+      HY_TRY(fft_c2r_full_api(g, f->W, f->w, st));
+      if (d.kind == HY_TERM_HARTREE) { a.hartree = 1; a.conv = f->w; }
+      else { a.ry = 1; a.wry = f->w; }
+    } else if (d.kind == HY_TERM_GRADIENT) {
+      for (int dd = 0; dd < 3; ++dd) {
+        double* dst = f->gn + nr * dd;
+        if (dd >= g.ndim) {
+          hipLaunchKernelGGL(k_fun_fill, dim3(nblocks(nr)), dim3(256), 0, st, dst, nr, 0.0);
+          continue;
+        }
+        const double* kdir = dd == 0 ? g.d_kx : (dd == 1 ? g.d_ky : g.d_kz);
+        const size_t kstride = dd == 0 ? 1 : (dd == 1 ? (size_t)g.nkx : (size_t)g.nkx * g.ny);
+        const int kn = dd == 0 ? g.nkx : (dd == 1 ? g.ny : g.nz);
+        hipLaunchKernelGGL(k_full_conv, dim3(nblocks(nf)), dim3(256), 0, st, fa, 1, kdir, kstride, kn, N,
+                           (const double*)nullptr, (const double*)nullptr, 0.0, f->W);
+        HY_TRY(fft_c2r_full_api(g, f->W, dst, st));
+      }
+      a.grad = 1; a.gn = f->gn;
+    } else if (d.kind == HY_TERM_IDEAL) {
+      a.ideal = 1;
+    } else {
+      a.tf = 1;
+    }
+    hipLaunchKernelGGL(k_fun_clear, dim3(1), dim3(1), 0, st, f->flag);
+    hipLaunchKernelGGL(k_point_budget, dim3(FUN_NB), dim3(256), 0, st, a);
+    HY_HIP(hipGetLastError());
+    std::vector<double> hp(FUN_NB * 8);
+    HY_HIP(hipMemcpyAsync(hp.data(), f->part, sizeof(double) * FUN_NB * 8, hipMemcpyDeviceToHost, st));
+    HY_TRY(fun_check(f, st));
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < FUN_NB; ++b)
+      for (int q = 0; q < 7; ++q) s[q] += hp[b * 8 + q];
+    const double V = g.volume, Nr = (double)nr;
+    switch (d.kind) {  // the terms' energy functions (fe_*.f90 energy)
+      case HY_TERM_IDEAL: total += (s[1] / Nr) * V / d.beta; break;
+      case HY_TERM_HARTREE: total += 0.5 * (s[2] / Nr) * V; break;
+      case HY_TERM_RY: total += 0.5 * (s[3] / Nr) * V; break;  // w holds B(-c_cor N' / beta)
+      case HY_TERM_TF: total += (s[4] / Nr) * V / d.beta; break;
+      case HY_TERM_GRADIENT: total += d.cg * (s[5] / Nr) * V; break;
+    }
+  }
+  *h_F = total;
   return HY_OK;
 }
 

@@ -116,6 +116,30 @@ typedef struct {
   const double* c_tab;      /* ry: host table c(k) (structure_source.f90:18-26)*/
 } hy_term_desc;
 
+/* ---- functional_t and its fe_term_t terms on FP64 device fields ---------
+   The reference's physics plug-in interface (free_energy.f90:22-59,
+   functional.f90:17-90) on a drop-in grid g, which must outlive the handle:
+   the terms' radial tables are copied at create.  d_n is a real field, d_nhat
+   its forward transform (hy_fft_r2c layout), d_mu a real field; stream ordered
+   on `stream` and synchronised at return (the density check).  A term index
+   is the position in `terms`; -1 in hy_fun_kernel / hy_fun_energy means the
+   whole functional.  HY_ERR_NONPOSITIVE where a term needs n > 0
+   (fe_ideal_classical.f90:39, fe_kinetic_tfk.f90:68,122); mu is then
+   unspecified. */
+typedef struct hy_fun_s* hy_fun;
+/* functional_add + functional_setup (functional.f90:32-53) */
+int hy_fun_create(hy_grid g, int nterms, const hy_term_desc* terms, hy_fun* out);
+int hy_fun_destroy(hy_fun f);
+int hy_fun_nterms(hy_fun f, int* n);
+/* fe_term_t%add_mu (free_energy.f90:39-46): mu += dF_term/dn */
+int hy_fun_add_mu(hy_fun f, int term, const double* d_n, const double* d_nhat, double* d_mu, void* stream);
+/* functional_mu (functional.f90:56-67): mu = 0, then every term's add_mu in list order */
+int hy_fun_mu(hy_fun f, const double* d_n, const double* d_nhat, double* d_mu, void* stream);
+/* fe_term_t%kernel (free_energy.f90:47-51), functional_kernel (functional.f90:70-78) */
+int hy_fun_kernel(hy_fun f, int term, double k, double* h_K);
+/* fe_term_t%energy (free_energy.f90:52-58), functional_energy (functional.f90:80-90) */
+int hy_fun_energy(hy_fun f, int term, const double* d_n, const double* d_nhat, double* h_F, void* stream);
+
 /* ---- model_t (src/model/model.f90) -------------------------------------- */
 #define HY_CLOSURE_NEWTONIAN 0
 #define HY_CLOSURE_MAXWELL 1
