@@ -121,6 +121,7 @@ _SIGS = {
     "hy_model_device_bytes": [_P, C.POINTER(C.c_ulonglong)],
     "hy_batch_create": [_P, _I, _I, _I, _PI, _I, C.POINTER(_P)],
     "hy_batch_launch": [_P, _I],
+    "hy_batch_set_stops": [_P, _P],
     "hy_batch_idle": [_P, _PI],
     "hy_batch_sync": [_P],
     "hy_batch_destroy": [_P],

@@ -170,6 +170,7 @@ struct PencilArgs {
   double dky; int ay_n;     // ky spacing; y stored count in the contiguous index (0 = none)
   int ay_h, ay_L;                // y stored-index mapping (h, L) when ay_n > 0
   int y_from_pencil;             // 1: ky comes from the pencil position (y pass)
+  const int* halt;               // every workgroup returns when *halt != 0 (null: never)
 };
 
 // signed index of FFT position p on an axis of length n (grid.f90:116-123)
@@ -272,6 +273,7 @@ struct PencilK {
 
 template <class T, int L>
 __device__ __attribute__((always_inline)) inline void pencil_inv_body(const PencilArgs& a, const cpx<T>* __restrict__ twg) {
+  if (gate_closed(a.halt)) return;  // uniform over the workgroup
   using S = PencilShape<T, L>;
   constexpr int L1 = S::L1, L2 = S::L2, TPC = S::TPC, NA = S::NA;
   const int al = threadIdx.x % NA, g = threadIdx.x / NA;
@@ -340,6 +342,7 @@ __device__ __attribute__((always_inline)) inline void pencil_inv_body(const Penc
 // Forward pass: in (A, L, B) -> out (A, K, B) kept entries, scaled by 1/L.
 template <class T, int L>
 __device__ __attribute__((always_inline)) inline void pencil_fwd_body(const PencilArgs& a, const cpx<T>* __restrict__ twg) {
+  if (gate_closed(a.halt)) return;  // uniform over the workgroup
   using S = PencilShape<T, L>;
   constexpr int L1 = S::L1, L2 = S::L2, TPC = S::TPC, NA = S::NA;
   const int al = threadIdx.x % NA, g = threadIdx.x / NA;
@@ -448,10 +451,11 @@ Recorder*& recorder() {
 }
 
 int pencil_pass(const Grid& g, int axis, bool inverse, bool trunc, int A, int B,
-                const PencilField* f, int nf, hipStream_t s) {
+                const PencilField* f, int nf, hipStream_t s, const int* halt) {
   if (nf <= 0) return HY_OK;
   PencilArgs a;
   std::memset(&a, 0, sizeof(a));
+  a.halt = halt;
   a.A = A;
   a.B = B;
   a.L = axis == 1 ? g.ny : g.nz;

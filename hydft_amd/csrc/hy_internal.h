@@ -283,8 +283,19 @@ constexpr int MAXPF = 24;
 //   inverse: in (A, Kp, B) truncated-or-full spectrum -> out (A, L, B)
 //   forward: in (A, L, B) -> out (A, Kp, B), scaled by 1/L
 // A is the contiguous count (elements), B the outer count.
+// `halt`: a device flag that makes every workgroup of the pass return at once
+// when set (a run that reached its step limit in a batch); null: never.
 int pencil_pass(const Grid& g, int axis, bool inverse, bool trunc, int A, int B,
-                const PencilField* f, int nf, hipStream_t s);
+                const PencilField* f, int nf, hipStream_t s, const int* halt = nullptr);
+
+// A device gate flag, read through the global address space.
+__device__ inline bool gate_closed(const int* h) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return h && *((const __attribute__((address_space(1))) int*)h) != 0;
+#else
+  return h && *h != 0;
+#endif
+}
 
 }  // namespace hy
 

@@ -101,6 +101,11 @@ struct Ctl {
   int step;
   int nonpos;                // set when a non-positive density is seen
   int nodt;                  // compute_dt could not find a time step
+  // step limit of a run in a batch of runs of different lengths: at the start
+  // of each recorded step sequence, halt = (stop > 0 && step >= stop), and
+  // every kernel of the sequence returns at once while halt is set
+  int stop;
+  int halt;
 };
 
 __constant__ double c_rk_a[3] = {32.0 / 60.0, 25.0 / 60.0, 45.0 / 60.0};  // model.f90:51-52
@@ -108,7 +113,7 @@ __constant__ double c_rk_b[3] = {0.0, -17.0 / 60.0, -25.0 / 60.0};
 static const double h_rk_a[3] = {32.0 / 60.0, 25.0 / 60.0, 45.0 / 60.0};
 static const double h_rk_b[3] = {0.0, -17.0 / 60.0, -25.0 / 60.0};
 
-enum CtlOp { CTL_STAGE_END = 0, CTL_STEP_END = 1, CTL_STEP_BEGIN = 2 };
+enum CtlOp { CTL_STAGE_END = 0, CTL_STEP_END = 1, CTL_STEP_BEGIN = 2, CTL_SEQ_BEGIN = 3, CTL_OPEN = 4 };
 struct CtlK {
   Ctl* c;
   int op, s;
@@ -121,6 +126,15 @@ __device__ __attribute__((always_inline)) inline void ctl_body(const CtlK& k) {
   const int op = k.op, s = k.s, evolve = k.evolve;
   const double tau = k.tau, tend = k.tend;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (op == CTL_SEQ_BEGIN) {
+    c->halt = (c->stop > 0 && c->step >= c->stop) ? 1 : 0;
+    return;
+  }
+  if (op == CTL_OPEN) {  // steps taken one model at a time ignore a batch's step limit
+    c->halt = 0;
+    return;
+  }
+  if (c->halt) return;
   if (op == CTL_STEP_BEGIN) {
     if (tend > 0.0 && c->t + c->dt > tend) c->dt = tend - c->t;  // src/hydft.f90:44
     c->ts = c->t;
@@ -141,6 +155,12 @@ __device__ __attribute__((always_inline)) inline void ctl_body(const CtlK& k) {
 }
 __global__ void k_ctl(CtlK k) { ctl_body(k); }
 __global__ void k_ctl_b(const CtlK* __restrict__ ks) { ctl_body(ks[blockIdx.z]); }
+__global__ void k_set_stop(Ctl* c, int stop) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    c->stop = stop;
+    c->halt = 0;
+  }
+}
 
 // ------------------------------------------------------------ global memory
 // (Device compilation; a host-side build of this file, such as a CPU
@@ -449,6 +469,7 @@ struct PhysArgs {
   constexpr int PTS = X::PTS;                                                          \
   __shared__ typename X::Smem sm;                                                      \
   const PhysArgs& a = *pa;                                                             \
+  if (gate_closed(&a.ctl->halt)) return;                                               \
   const int lane = threadIdx.x;                                                        \
   const int lw = lane / X::TPC, g = lane % X::TPC;                                     \
   const size_t line = (size_t)xcd_work_index(blockIdx.x, gridDim.x) * X::LPW + lw;     \
@@ -706,6 +727,7 @@ __device__ __attribute__((always_inline)) inline void dt_body(const DtK& k) {
   const int nlines = k.nlines;
   Ctl* c = k.c;
   const double wmax = k.wmax, numax = k.numax, dxmin = k.dxmin, cfl = k.cfl;
+  if (gate_closed(&c->halt)) return;  // uniform over the block
   __shared__ double sh[256];
   double m = 0.0;
   for (int i = threadIdx.x; i < nlines; i += blockDim.x) m = fmax(m, red[i]);
@@ -736,12 +758,14 @@ struct RedK {
   int nlines, ncomp;
   double scale;
   double* out;
+  const int* halt;
 };
 __device__ __attribute__((always_inline)) inline void reduce_lines_body(const RedK& k) {
   const double* __restrict__ red = k.red;
   const int nlines = k.nlines, ncomp = k.ncomp;
   const double scale = k.scale;
   double* __restrict__ out = k.out;
+  if (gate_closed(k.halt)) return;  // uniform over the block
   __shared__ double sh[256];
   for (int c = 0; c < ncomp; ++c) {
     double s = 0.0;
@@ -817,6 +841,7 @@ struct K1Args {
 
 template <class T>
 __device__ __attribute__((always_inline)) inline void spec1_body(const K1Args& a) {
+  if (gate_closed(&a.ctl->halt)) return;
   const size_t nt = a.s.nt;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nt; e += (size_t)gridDim.x * blockDim.x) {
     c64 nh = gldc<double>(a.N, e);
@@ -872,6 +897,7 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
   const size_t nt = a.s.nt;
   constexpr int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
   constexpr int idx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};  // stress.f90:92
+  if (gate_closed(&a.ctl->halt)) return;
   const double dt = a.ctl->dt;
   const double ra = c_rk_a[a.stage], rb = c_rk_b[a.stage];
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nt; e += (size_t)gridDim.x * blockDim.x) {
@@ -986,6 +1012,7 @@ __device__ __attribute__((always_inline)) inline void probe_ring_body(const Prob
   double* out = k.out;
   const int ring = k.ring;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (c->halt) return;
   const int slot = (int)(((long long)c->step - 1) % ring + ring) % ring;
   c64 a = idx >= 0 ? S[idx] : mk<double>(0, 0);
   c64 b = idx >= 0 ? S[nt + idx] : mk<double>(0, 0);
@@ -1181,11 +1208,11 @@ static int inv_chain(Model& m, const std::vector<InvSpec>& fs, std::vector<XIn>&
     }
     if ((int)fs.size() > m.nzslots) return set_error(HY_ERR_INVALID, "internal: z slots");
     prof_begin(m, PC_ZINV, pencil_bytes(m, zf, (size_t)g.Kx * g.Ky, 1, g.nz, g.Kz));
-    HY_TRY(pencil_pass(g, 2, true, true, g.Kx * g.Ky, 1, zf.data(), (int)zf.size(), m.st));
+    HY_TRY(pencil_pass(g, 2, true, true, g.Kx * g.Ky, 1, zf.data(), (int)zf.size(), m.st, &m.ctl->halt));
     prof_end(m);
     if (g.ny > 1) {
       prof_begin(m, PC_YINV, pencil_bytes(m, yf, g.Kx, g.nz, g.ny, g.Ky));
-      HY_TRY(pencil_pass(g, 1, true, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st));
+      HY_TRY(pencil_pass(g, 1, true, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st, &m.ctl->halt));
       prof_end(m);
     }
     for (auto& f : fs) xin.push_back({xslot(m, f.xslot), 0, f.mx, 0.0});
@@ -1196,7 +1223,7 @@ static int inv_chain(Model& m, const std::vector<InvSpec>& fs, std::vector<XIn>&
     std::vector<PencilField> yf;
     for (auto& f : fs) yf.push_back({f.src, xslot(m, f.xslot), f.my, f.src64 ? 1 : ct64(m), f.k0off});
     prof_begin(m, PC_YINV, pencil_bytes(m, yf, g.Kx, 1, g.ny, g.Ky));
-    HY_TRY(pencil_pass(g, 1, true, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st));
+    HY_TRY(pencil_pass(g, 1, true, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st, &m.ctl->halt));
     prof_end(m);
     for (auto& f : fs) xin.push_back({xslot(m, f.xslot), 0, f.mx, 0.0});
   } else {
@@ -1224,17 +1251,17 @@ static int fwd_chain(Model& m, const std::vector<std::pair<int, int>>& f /* (xsl
     if ((int)f.size() > m.nzslots) return set_error(HY_ERR_INVALID, "internal: z slots");
     if (g.ny > 1) {
       prof_begin(m, PC_YFWD, pencil_bytes(m, yf, g.Kx, g.nz, g.ny, g.Ky));
-      HY_TRY(pencil_pass(g, 1, false, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st));
+      HY_TRY(pencil_pass(g, 1, false, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st, &m.ctl->halt));
       prof_end(m);
     }
     prof_begin(m, PC_ZFWD, pencil_bytes(m, zf, (size_t)g.Kx * g.Ky, 1, g.nz, g.Kz));
-    HY_TRY(pencil_pass(g, 2, false, true, g.Kx * g.Ky, 1, zf.data(), (int)zf.size(), m.st));
+    HY_TRY(pencil_pass(g, 2, false, true, g.Kx * g.Ky, 1, zf.data(), (int)zf.size(), m.st, &m.ctl->halt));
     prof_end(m);
   } else if (g.ny > 1) {
     std::vector<PencilField> yf;
     for (auto& p : f) yf.push_back({xslot(m, p.first), fslot(m, p.second), MUL_NONE, ct64(m), 0.0});
     prof_begin(m, PC_YFWD, pencil_bytes(m, yf, g.Kx, 1, g.ny, g.Ky));
-    HY_TRY(pencil_pass(g, 1, false, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st));
+    HY_TRY(pencil_pass(g, 1, false, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st, &m.ctl->halt));
     prof_end(m);
   }
   return HY_OK;
@@ -1448,7 +1475,7 @@ static int stage(Model& m, int s, int rhs_only) {
   HY_TRY(run_xk(m, XK_B, XB_STAGE, b));
   HY_TRY(fwd_chain(m, fwdb));
   if (m.vext_kind != HY_VEXT_NONE) {
-    const RedK rk{m.red, g.ny * g.nz, 3, 1.0 / (double)g.nreal(), m.ctl->nf};
+    const RedK rk{m.red, g.ny * g.nz, 3, 1.0 / (double)g.nreal(), m.ctl->nf, &m.ctl->halt};
     HY_TRY(launch(HY_FN(k_reduce_lines), HY_FN(k_reduce_lines_b), dim3(1), dim3(256), m.st, rk));
   }
   // ---- K2: RHS assembly and the RK update
@@ -1557,7 +1584,7 @@ struct SetDtK {
   double dt;
 };
 __device__ __attribute__((always_inline)) inline void set_dt_body(const SetDtK& k) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) k.c->dt = k.dt;
+  if (threadIdx.x == 0 && blockIdx.x == 0 && !k.c->halt) k.c->dt = k.dt;
 }
 __global__ void k_set_dt(SetDtK k) { set_dt_body(k); }
 __global__ void k_set_dt_b(const SetDtK* __restrict__ ks) { set_dt_body(ks[blockIdx.z]); }
@@ -2242,6 +2269,7 @@ int hy_model_set_dt(hy_model h, double dt) {
 int hy_model_compute_dt(hy_model h, double* h_dt) {
   if (!h) return set_error(HY_ERR_INVALID, "null argument");
   Model& m = h->m;
+  HY_TRY(ctl_op(m, CTL_OPEN, 0, 0.0));
   HY_TRY(finish_relax(m));
   HY_TRY(compute_dt_dev(m));
   Ctl c;
@@ -2255,6 +2283,7 @@ int hy_model_advance(hy_model h, int nsteps, int dt_every, double tend, const in
                      double* h_probe, int* h_taken) {
   if (!h || nsteps < 0) return set_error(HY_ERR_INVALID, "hy_model_advance: bad argument");
   Model& m = h->m;
+  HY_TRY(ctl_op(m, CTL_OPEN, 0, 0.0));
   const Grid& g = m.g;
   long long pidx = -1;
   if (probe3) {
@@ -2363,6 +2392,7 @@ static int prepare_steps(Model& m, int nsteps, int dt_every, const int* probe3, 
 // takes them, each step followed by its probe row into the ring; ends with
 // no pending relaxation and at the start of the dt cadence again.
 static int emit_steps(Model& m, int nsteps, int dt_every, long long pidx, bool probe, int ring) {
+  HY_TRY(ctl_op(m, CTL_SEQ_BEGIN, 0, 0.0));
   for (int i = 0; i < nsteps; ++i) {
     if (dt_every > 0) {
       if (m.irecompute == 0) {
@@ -2482,13 +2512,19 @@ struct hy_batch_s {
   std::vector<Item> items;
   void* d_all = nullptr;
   bool holds = false;  // the models count this batch in their nbatch
+  bool stops_set = false;
+  int nsteps = 0;      // steps per launch
 };
 
 static void batch_free(hy_batch_s* b) {
   if (!b) return;
-  if (b->st) (void)hipStreamSynchronize(b->st);
-  if (b->holds)
+  if (b->holds) {
+    // the models run on alone afterwards: no step limit
+    if (b->stops_set)
+      for (hy_model h : b->models) hipLaunchKernelGGL(k_set_stop, dim3(1), dim3(1), 0, b->st, h->m.ctl, 0);
     for (hy_model h : b->models) --h->m.nbatch;
+  }
+  if (b->st) (void)hipStreamSynchronize(b->st);
   if (b->d_all) (void)hipFree(b->d_all);
   if (b->st) (void)hipStreamDestroy(b->st);
   delete b;
@@ -2589,6 +2625,7 @@ int hy_batch_create(const hy_model* ms, int n, int nsteps, int dt_every, const i
   }
   for (hy_model h : b->models) ++h->m.nbatch;
   b->holds = true;
+  b->nsteps = nsteps;
   *out = b;
   return HY_OK;
 }
@@ -2608,6 +2645,21 @@ int hy_batch_launch(hy_batch b, int nlaunch) {
       }
       if (e != hipSuccess) return set_error(HY_ERR_HIP, std::string("hy_batch_launch: ") + hipGetErrorString(e));
     }
+  return HY_OK;
+}
+
+int hy_batch_set_stops(hy_batch b, const int* stops) {
+  if (!b || !stops) return set_error(HY_ERR_INVALID, "hy_batch_set_stops: null argument");
+  for (size_t i = 0; i < b->models.size(); ++i)
+    if (stops[i] < 0 || stops[i] % b->nsteps != 0)
+      return set_error(HY_ERR_INVALID, "hy_batch_set_stops: stop " + std::to_string(stops[i]) + " of model " +
+                                           std::to_string(i) + " is not a non-negative multiple of the batch's " +
+                                           std::to_string(b->nsteps) + " steps per launch");
+  // stream ordered after the launches already queued
+  for (size_t i = 0; i < b->models.size(); ++i)
+    hipLaunchKernelGGL(k_set_stop, dim3(1), dim3(1), 0, b->st, b->models[i]->m.ctl, stops[i]);
+  HY_HIP(hipGetLastError());
+  b->stops_set = true;
   return HY_OK;
 }
 
@@ -2650,6 +2702,7 @@ int hy_model_probe_ring(hy_model h, int ring, double* h_probe) {
 int hy_model_rhs(hy_model h, double t, double* h_dn, double* h_dmom3, double* h_dpi6) {
   if (!h) return set_error(HY_ERR_INVALID, "null argument");
   Model& m = h->m;
+  HY_TRY(ctl_op(m, CTL_OPEN, 0, 0.0));
   HY_TRY(finish_relax(m));
   Ctl c;
   HY_TRY(read_ctl(m, c));

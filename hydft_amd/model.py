@@ -372,6 +372,12 @@ class Batch:
     def launch(self, nlaunch=1):
         abi.call("hy_batch_launch", self.h, int(nlaunch))
 
+    def set_stops(self, stops):
+        """Per-model step limits (multiples of nsteps; 0: none): a run stops there
+        while the others go on."""
+        arr = (C.c_int * len(self.models))(*[int(v) for v in stops])
+        abi.call("hy_batch_set_stops", self.h, arr)
+
     def idle(self):
         v = C.c_int()
         abi.call("hy_batch_idle", self.h, C.byref(v))

@@ -247,6 +247,14 @@ typedef struct hy_batch_s* hy_batch;
 int hy_batch_create(const hy_model* models, int n, int nsteps, int dt_every, const int* probe3, int ring,
                     hy_batch* out);
 int hy_batch_launch(hy_batch b, int nlaunch);
+/* Runs of different lengths in one batch: model i takes no further step once
+   its step count reaches stops[i] (0: no limit); every kernel of its later
+   launches returns at once, so its state, time and probe ring stay those after
+   step stops[i], exactly as when it runs alone.  A stop must be a multiple of
+   the batch's nsteps (steps per launch), counted from step 0; the limits are
+   stream ordered after the launches already queued and are lifted when the
+   batch is destroyed. */
+int hy_batch_set_stops(hy_batch b, const int* stops);
 int hy_batch_idle(hy_batch b, int* idle);
 int hy_batch_sync(hy_batch b);
 int hy_batch_destroy(hy_batch b);

@@ -137,3 +137,47 @@ def test_batches_split_by_kept_box(gpu):
     a = {r.rid: r for r in serve.one_at_a_time(pick, precision="fp64")}
     for rid in a:
         assert np.array_equal(a[rid].probe, b[rid].probe), rid
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,closure", [("fp64", "newtonian"), ("mixed", "maxwell")])
+def test_batch_runs_of_different_lengths(gpu, precision, closure):
+    """Runs of 20, 40 and 60 steps in one batch (hy_batch_set_stops): each ends
+    bitwise where it ends alone (state, probe rows, time), while the batch
+    takes 60 steps; afterwards a stopped run continues alone as usual."""
+    reqs = _reqs(3, closure=closure, seed=320)
+    lengths = [20, 60, 40]
+    cache = serve._hnc.StructureCache()
+    alone = []
+    for r, n in zip(reqs, lengths):
+        m = serve.make_model(r, precision, cache)
+        taken, rows = m.advance(n, dt_every=20, probe=r.mode)
+        assert taken == n
+        alone.append((m.get_state(), rows, m.time))
+        m.close()
+    ms = [serve.make_model(r, precision, cache) for r in reqs]
+    b = Batch(ms, 20, dt_every=20, probe=(1, 0, 0), ring=60)
+    b.set_stops(lengths)
+    b.launch(3)
+    b.sync()
+    for i, (m, n) in enumerate(zip(ms, lengths)):
+        (na, ma, pa), rows, t = alone[i]
+        nb, mb, pb = m.get_state()
+        assert np.array_equal(na, nb) and np.array_equal(ma, mb), i
+        if pa is not None:
+            assert np.array_equal(pa, pb), i
+        assert np.array_equal(rows, b.probe(i)[:n]), i
+        assert m.time == t, i
+    with pytest.raises(abi.HydftError):
+        b.set_stops([10, 20, 20])
+    b.close()
+    # the 20-step run continues alone: 20 more steps equal a 40-step run alone
+    m = ms[0]
+    m.advance(20, dt_every=20)
+    ref = serve.make_model(reqs[0], precision, cache)
+    ref.advance(40, dt_every=20)
+    assert np.array_equal(m.get_state()[0], ref.get_state()[0])
+    assert m.time == ref.time
+    ref.close()
+    for m in ms:
+        m.close()
