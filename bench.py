@@ -183,126 +183,149 @@ def kernel_traffic(tr, cls, precision):
     return None
 
 
-def serve_sample(nreq, steps, grids=(32, 48, 64)):
-    """Batched / served chi(k, w) throughput against one run at a time, on a
-    seeded sample of driven-mode requests in the form of the batched scan:
-    Yukawa ions, grids cycling through 32^3, 48^3, 64^3, Gamma log-uniform in
-    [1, 100], kappa in [0.5, 3], Newtonian / Maxwell alternating, drive
-    1e-4 cos(q x - w t), CFL dt; FP32-mixed.  HNC c(k) are solved (in parallel
-    host processes) before the timed parts, except in the cold-cache pair,
-    which includes every request's c(k) solve."""
+def _one_at_a_time_estimate(reqs, cache, per_class=2):
+    """The reference's way (one model at a time, hy_model_advance) timed on a
+    stratified sample: `per_class` requests of each (grid, species, closure)
+    class, scaled by the class counts.  Returns (estimated seconds for all
+    requests, sample size, the sample's results)."""
+    from hydft_amd import serve
+    classes = {}
+    for r in reqs:
+        classes.setdefault((r.p.nx, r.p.kind, r.p.closure), []).append(r)
+    total, n, results = 0.0, 0, []
+    for key, rs in classes.items():
+        pick = rs[:per_class]
+        t0 = time.perf_counter()
+        res = serve.one_at_a_time(pick, precision="mixed", cache=cache)
+        el = time.perf_counter() - t0
+        steps_pick = sum(r.nsteps for r in pick)
+        steps_all = sum(r.nsteps for r in rs)
+        total += el * steps_all / steps_pick
+        n += len(pick)
+        results += res
+    return total, n, results
+
+
+def scan_sample(nruns, steps, grids=(32, 48, 64)):
+    """The batched driven-mode scan: `nruns` Yukawa-ion chi(k, w) runs, grids
+    cycling through 32^3, 48^3, 64^3, Gamma log-uniform in [1, 100], kappa in
+    [0.5, 3], Newtonian / Maxwell alternating (tau = 1), drive 1e-4 cos(q x - w t)
+    with q in [0.2, 2] and w in [0.1, 3] w_p, CFL dt every 20 steps, `steps`
+    steps each, FP32-mixed, all submitted at once.  The c(k) of all state points
+    are solved together on the GPU first (timed apart: `hnc_device_s`).  Warm
+    cache: batched (one launch per kernel for each batch of same-shape runs)
+    against one at a time, the latter timed on a stratified sample.  Cold cache,
+    the reference's way on a small sample: host HNC then the run, one at a
+    time."""
     from hydft_amd import serve
     from hydft_amd.host import hnc
     from hydft_amd.model import hnc_request
-    reqs = serve.chi_requests(nreq, seed=11, grid=grids, steps=steps)
+    reqs = serve.chi_requests(nruns, seed=11, grid=grids, steps=steps)
     cache = hnc.StructureCache()
     t0 = time.perf_counter()
-    cache.solve_many([hnc_request(r.p) for r in reqs])
+    cache.solve_many([hnc_request(r.p) for r in reqs], device=True)
     t_hnc = time.perf_counter() - t0
-    print("# serve sample: hnc presolve done", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    one = serve.one_at_a_time(reqs, precision="mixed", cache=cache)
-    w_one = time.perf_counter() - t0
-    print("# serve sample: one at a time done", file=sys.stderr, flush=True)
-    # concurrent runs, one stream and one captured graph per run
-    srv_st = serve.Server(precision="mixed", cache=cache, max_slots=64)
-    t0 = time.perf_counter()
-    streams = srv_st.serve(reqs, realtime=False)
-    w_streams = time.perf_counter() - t0
-    print("# serve sample: streams done", file=sys.stderr, flush=True)
-    # batches: same-shape runs share every launch
-    srv = serve.Server(precision="mixed", cache=cache, max_slots=64, batch=True)
+    print(f"# scan: {nruns} c(k) on the GPU in {t_hnc:.1f} s", file=sys.stderr, flush=True)
+    srv = serve.Server(precision="mixed", cache=cache, max_slots=64, batch=True, max_batch=512)
     t0 = time.perf_counter()
     batch = srv.serve(reqs, realtime=False)
     w_batch = time.perf_counter() - t0
-    print("# serve sample: batches done", file=sys.stderr, flush=True)
-    # served stream: Poisson arrivals, batches formed from what has arrived
-    rate = 50.0
-    reqs_rt = serve.chi_requests(nreq, seed=11, grid=grids, steps=steps, arrival_rate=rate)
-    srv_rt = serve.Server(precision="mixed", cache=cache, max_slots=64, batch=True)
-    t0 = time.perf_counter()
-    served = srv_rt.serve(reqs_rt, realtime=True)
-    w_served = time.perf_counter() - t0
-    print("# serve sample: served stream done", file=sys.stderr, flush=True)
-
-    # cold c(k) cache, end to end: every request needs its own HNC solve.  The
-    # reference's way is a host solve then the run, one request at a time
-    # (timed on a bounded sample); the server solves the c(k) of all arrived
-    # requests together on the GPU, then runs them in batches.
-    ncold = min(8, nreq)
+    print(f"# scan: batched {nruns} runs in {w_batch:.1f} s", file=sys.stderr, flush=True)
+    w_one, n_one, one = _one_at_a_time_estimate(reqs, cache)
+    print(f"# scan: one at a time sample of {n_one} done", file=sys.stderr, flush=True)
+    by = {r.rid: r for r in batch}
+    same = all(np.array_equal(r.probe, by[r.rid].probe) for r in one)
+    # cold cache, the reference's way: host HNC solve, then the run
+    ncold = 4
     cache_h = hnc.StructureCache()
     t0 = time.perf_counter()
-    one_cold = serve.one_at_a_time(reqs[:ncold], precision="mixed", cache=cache_h)
-    w_one_cold = (time.perf_counter() - t0) / ncold
-    print("# serve sample: cold one at a time done", file=sys.stderr, flush=True)
-    srv_c = serve.Server(precision="mixed", cache=hnc.StructureCache(), max_slots=64, batch=True, hnc_device=True)
-    t0 = time.perf_counter()
-    served_cold = srv_c.serve(reqs, realtime=False)
-    w_cold = time.perf_counter() - t0
-    print("# serve sample: cold served done", file=sys.stderr, flush=True)
-    cold_by = {r.rid: r for r in served_cold}
-    chi_diff = max(abs(cold_by[r.rid].value["chi"] / r.value["chi"] - 1.0) for r in one_cold)
-
-    def _same(xs):
-        return all(np.array_equal(a.probe, b.probe) for a, b in
-                   zip(sorted(one, key=lambda r: r.rid), sorted(xs, key=lambda r: r.rid)))
-
-    s_one, s_b = serve.summary(one, w_one), serve.summary(batch, w_batch)
-    s_st, s_s = serve.summary(streams, w_streams), serve.summary(served, w_served)
-    return {"requests": nreq, "grids": list(grids), "steps_per_run": steps, "hnc_presolve_s": t_hnc,
-            "one_at_a_time_runs_per_s": s_one["runs_per_s"],
-            "streams_runs_per_s": s_st["runs_per_s"],
+    serve.one_at_a_time(reqs[:ncold], precision="mixed", cache=cache_h)
+    w_cold = (time.perf_counter() - t0) / ncold
+    s_b = serve.summary(batch, w_batch)
+    return {"runs": nruns, "grids": list(grids), "steps_per_run": steps,
+            "hnc_device_s": t_hnc,
             "batched_runs_per_s": s_b["runs_per_s"],
-            "batched_over_one_at_a_time": s_b["runs_per_s"] / s_one["runs_per_s"],
             "batched_launch_groups": srv.batches_launched,
-            "batched_latency_p50_s": s_b["latency_p50_s"], "batched_latency_p99_s": s_b["latency_p99_s"],
+            "one_at_a_time_runs_per_s": nruns / w_one,
+            "one_at_a_time_sample": n_one,
+            "batched_over_one_at_a_time": s_b["runs_per_s"] * w_one / nruns,
+            "cold_ck_one_at_a_time_runs_per_s": 1.0 / w_cold, "cold_ck_sample": ncold,
+            "cold_ck_batched_runs_per_s": nruns / (w_batch + t_hnc),
+            "cold_ck_batched_over_one_at_a_time": (nruns / (w_batch + t_hnc)) * w_cold,
+            "sample_results_equal_batched": bool(same),
+            "chi_vs_linear_median": s_b.get("chi_vs_linear_median")}
 
-            "served_arrival_rate_per_s": rate, "served_runs_per_s": s_s["runs_per_s"],
-            "served_launch_groups": srv_rt.batches_launched,
-            "served_latency_p50_s": s_s["latency_p50_s"], "served_latency_p99_s": s_s["latency_p99_s"],
-            "one_at_a_time_latency_p50_s": s_one["latency_p50_s"],
-            "results_equal_one_at_a_time": bool(_same(batch) and _same(streams) and _same(served)),
-            "cold_ck_one_at_a_time_runs_per_s": 1.0 / w_one_cold, "cold_ck_one_at_a_time_sample": ncold,
-            "cold_ck_served_runs_per_s": nreq / w_cold,
-            "cold_ck_served_over_one_at_a_time": (nreq / w_cold) * w_one_cold,
-            "cold_ck_chi_max_rel_diff": chi_diff}
+
+def stream_sample(nreq, rate=50.0):
+    """The serving stream: `nreq` chi(k, w) / dispersion requests with Poisson
+    arrivals at `rate`/s (hydft_amd.serve.stream_requests: grids 32/64/96/128^3
+    at weights .4/.3/.2/.1, ions or electrons, Newtonian or Maxwell, 500-5000
+    steps, FP32-mixed), served with continuous batching: requests that have
+    arrived are admitted in batches as slots and HBM free up while earlier ones
+    run.  The c(k) are solved on the GPU before the stream starts (timed
+    apart).  Reports runs/s and p50/p99 latency (arrival to result); one at a
+    time is timed on a stratified sample."""
+    from hydft_amd import serve
+    from hydft_amd.host import hnc
+    from hydft_amd.model import hnc_request
+    reqs = serve.stream_requests(nreq, seed=5, arrival_rate=rate)
+    cache = hnc.StructureCache()
+    t0 = time.perf_counter()
+    cache.solve_many([hnc_request(r.p) for r in reqs], device=True)
+    t_hnc = time.perf_counter() - t0
+    print(f"# stream: {nreq} c(k) on the GPU in {t_hnc:.1f} s", file=sys.stderr, flush=True)
+    srv = serve.Server(precision="mixed", cache=cache, max_slots=64, batch=True, max_batch=256)
+    t0 = time.perf_counter()
+    served = srv.serve(reqs, realtime=True)
+    w = time.perf_counter() - t0
+    print(f"# stream: {nreq} requests served in {w:.1f} s", file=sys.stderr, flush=True)
+    w_one, n_one, one = _one_at_a_time_estimate(reqs, cache, per_class=1)
+    by = {r.rid: r for r in served}
+    same = all(np.array_equal(r.probe, by[r.rid].probe) for r in one)
+    s = serve.summary(served, w)
+    mix = {}
+    for r in reqs:
+        k = f"{r.p.nx}^3 {r.p.kind}"
+        mix[k] = mix.get(k, 0) + 1
+    return {"requests": nreq, "arrival_rate_per_s": rate, "mix": mix,
+            "mean_steps": float(np.mean([r.nsteps for r in reqs])),
+            "hnc_device_s": t_hnc,
+            "served_runs_per_s": s["runs_per_s"], "latency_p50_s": s["latency_p50_s"],
+            "latency_p99_s": s["latency_p99_s"], "launch_groups": srv.batches_launched,
+            "one_at_a_time_runs_per_s": nreq / w_one, "one_at_a_time_sample": n_one,
+            "served_over_one_at_a_time": s["runs_per_s"] * w_one / nreq,
+            "sample_results_equal_served": bool(same)}
 
 
 def maxwell_sample(n=128, steps=1000):
     """The 128^3 Maxwell configuration: Yukawa ions, Gamma = 50, kappa = 2,
     Maxwell closure with Pi as state (Stanton-Murillo eta, Ichimaru tau from
     the HNC excess energy), dt = 0.005, 1000 steps, FP64 and FP32-mixed from
-    the same state.  Initial dn: a seeded Gaussian random field with spectral
-    shape exp(-q^2/2) and random phases, zero mean, scaled to an rms of
-    1e-2 n0 (an amplitude of 1e-2 n0 on every mode would drive the density
-    negative at this box size); u = 0, Pi = 0.  Returns steps/s of both and
+    the same state.  Initial dn: a seeded Gaussian random field with per-mode
+    modulus 1e-2 n0 exp(-q^2/2) and uniform random phases, zero mean, in a box
+    of 4 pi (a box of 2 pi/0.2 would put an rms of 0.26 n0 on this spectrum
+    and the density below zero); u = 0, Pi = 0.  Returns steps/s of both and
     the mixed density error against FP64 after the run."""
     from hydft_amd import abi
+    from hydft_amd.host.initial import gauss_spectrum_field
     from hydft_amd.host.params import Params
     from hydft_amd.model import Model, build_species
     p = Params()
     p.kind, p.gamma, p.kappa = "ion", 50.0, 2.0
     p.nx = p.ny = p.nz = n
-    p.lx = p.ly = p.lz = 2.0 * math.pi / 0.2
+    p.lx = p.ly = p.lz = 4.0 * math.pi
     p.closure, p.transport, p.tau_model = "maxwell", "stanton_murillo", "ichimaru"
     p.dt = 0.005
     p.finalize()
     sp = build_species(p)
-    rng = np.random.default_rng(2024)
-    w = rng.standard_normal((n, n, n))
-    kx = 2.0 * math.pi * np.fft.rfftfreq(n, p.lx / n)
-    ky = 2.0 * math.pi * np.fft.fftfreq(n, p.ly / n)
-    kz = 2.0 * math.pi * np.fft.fftfreq(n, p.lz / n)
-    q2 = kz[:, None, None] ** 2 + ky[None, :, None] ** 2 + kx[None, None, :] ** 2
-    dn = np.fft.irfftn(np.fft.rfftn(w) * np.exp(-0.5 * q2), s=(n, n, n))
-    dn -= dn.mean()
     n0 = sp.units.n0
-    dn *= 1e-2 * n0 / np.sqrt(np.mean(dn * dn))
-    out = {"grid": n, "steps": steps, "dt": p.dt}
+    dens = gauss_spectrum_field(p, n0, 1e-2, 1234)
+    out = {"grid": n, "steps": steps, "dt": p.dt, "box": p.lx,
+           "dn_rms_over_n0": float(np.sqrt(np.mean((dens - n0) ** 2)) / n0)}
     fields = {}
     for prec in ("fp64", "mixed"):
         m = Model(p, species=sp, precision=prec)
-        m.set_state(n0 + dn)
+        m.set_state(dens)
         m.advance(5)
         abi.call("hy_device_sync")
         t0 = time.perf_counter()
@@ -316,6 +339,24 @@ def maxwell_sample(n=128, steps=1000):
     out["mixed_density_rel_l2_vs_fp64"] = float(np.linalg.norm(b - a) / np.linalg.norm(a))
     out["mixed_dn_rel_l2_vs_fp64"] = float(np.linalg.norm(b - a) / np.linalg.norm(a - n0))
     return out
+
+
+def mixed_error_run(p, species, n, mom, dt, steps=500):
+    """Density error of the FP32-mixed run against the FP64 run after `steps`
+    steps of the benchmark configuration (the configuration's 500)."""
+    from hydft_amd.model import Model
+    out = {}
+    for prec in ("fp64", "mixed"):
+        m = Model(p, species=species, precision=prec)
+        m.set_state(n, mom)
+        m.set_dt(dt)
+        m.advance(steps)
+        out[prec] = m.get_state()[0]
+        m.close()
+    a, b = out["fp64"], out["mixed"]
+    n0 = species.units.n0
+    return {"steps": steps, "density_rel_l2": float(np.linalg.norm(b - a) / np.linalg.norm(a)),
+            "dn_rel_l2": float(np.linalg.norm(b - a) / np.linalg.norm(a - n0))}
 
 
 def batch_only(spec, steps):
@@ -382,10 +423,15 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates HBM byte counters)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter runs")
-    ap.add_argument("--serve", type=int, default=96, metavar="NREQ",
-                    help="batched/served chi(k, w) sample size (0: skip)")
-    ap.add_argument("--serve-steps", type=int, default=400)
+    ap.add_argument("--scan", type=int, default=512, metavar="NRUNS",
+                    help="batched driven-mode scan: number of runs (0: skip)")
+    ap.add_argument("--scan-steps", type=int, default=2000)
+    ap.add_argument("--stream", type=int, default=100, metavar="NREQ",
+                    help="serving stream: number of requests (0: skip)")
+    ap.add_argument("--serve-steps", type=int, default=400, help="steps per run of --batch-only")
     ap.add_argument("--no-maxwell", action="store_true", help="skip the 128^3 Maxwell FP64/mixed sample")
+    ap.add_argument("--error-steps", type=int, default=500,
+                    help="steps of the mixed-vs-FP64 density error run (0: skip)")
     ap.add_argument("--hnc-only", type=int, default=0, metavar="NSTATE",
                     help="only solve NSTATE state points of the scan with the batched GPU HNC, and time it")
     ap.add_argument("--batch-only", default="", metavar="B,N,CLOSURE",
@@ -468,8 +514,12 @@ def main():
         extra["mixed_dn_rel_l2_vs_fp64"] = float(np.linalg.norm(nm - n64) / np.linalg.norm(n64 - m.n0))
         extra["steps_compared"] = args.warmup + args.steps
         m64.close()
-    if args.serve > 0:
-        extra["serve"] = serve_sample(args.serve, args.serve_steps)
+        if args.error_steps > 0:
+            extra["mixed_vs_fp64_long"] = mixed_error_run(p, un_sp, n, mom, dt, args.error_steps)
+    if args.scan > 0:
+        extra["scan"] = scan_sample(args.scan, args.scan_steps)
+    if args.stream > 0:
+        extra["stream"] = stream_sample(args.stream)
     if not args.no_maxwell:
         extra["maxwell_128"] = maxwell_sample()
     cpu = None

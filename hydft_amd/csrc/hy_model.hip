@@ -1348,7 +1348,29 @@ static int xargs_dev(Model& m, int slot, const PhysArgs& a, const PhysArgs** out
   *out = m.d_xargs + slot;
   return HY_OK;
 }
+// Every input an x-line kernel will read must be set: a missing one would be
+// read through a null pointer.
+static int check_xk_inputs(const Model& m, int which, const PhysArgs& a) {
+  const int nd = m.g.ndim;
+  bool ok = a.n.p != nullptr && a.ctl != nullptr;
+  for (int d = 0; d < nd; ++d) ok = ok && a.m[d].p != nullptr;
+  if (which == XK_A && a.grad) {
+    ok = ok && a.gx.p && a.lap.p && (nd < 2 || a.gy.p) && (nd < 3 || a.gz.p) && a.fdtab;
+  }
+  if (which == XK_A && a.tf) ok = ok && a.fdtab;
+  if (which == XK_B) {
+    ok = ok && a.mux.p && (nd < 2 || a.muy.p) && (nd < 3 || a.muz.p);
+    if (a.evolve)
+      for (int c = 0; c < 6; ++c) ok = ok && a.px[c].p && (nd < 2 || a.py[c].p) && (nd < 3 || a.pz[c].p);
+  }
+  if (which != XK_UMAX)
+    for (int o = 0; o < a.nout; ++o) ok = ok && a.dst[o] != nullptr;
+  if (!ok) return set_error(HY_ERR_INVALID, "internal: an x-line kernel input or output is not set");
+  return HY_OK;
+}
+
 static int run_xk(Model& m, int which, int slot, const PhysArgs& a) {
+  HY_TRY(check_xk_inputs(m, which, a));
   const TwiddleSet& t = m.g.tw(m.g.nx);
   const PhysArgs* da = nullptr;
   HY_TRY(xargs_dev(m, slot, a, &da));
@@ -1562,6 +1584,11 @@ static int finish_relax(Model& m) {
   const int nd = m.g.ndim;
   PhysArgs a = phys_base(m);
   HY_TRY(stage_inputs(m, a, false));
+  // only u = mom / (m n) is formed here: no functional, so none of its inputs
+  // (the gradient term's grad n and lap n are not loaded on this path)
+  a.ideal = 0;
+  a.tf = 0;
+  a.grad = 0;
   std::vector<std::pair<int, int>> fwd;
   a.nout = 0;
   for (int d = 0; d < nd; ++d) {

@@ -21,9 +21,10 @@ what else runs beside it.
 
 With `batch=True` the server admits the arrived requests in groups of the
 same shape: same grid, species kind, closure, drive kind, probed mode, time
-step policy and step count, split further by the models' kept box (for N
-divisible by 3 the 2/3-rule boundary mode depends on the box length's
-rounding).  A group runs as one batch (hy_batch_*): each
+step policy and step counts within a factor of two, split further by the
+models' kept box (for N divisible by 3 the 2/3-rule boundary mode depends on
+the box length's rounding).  A batch stops each run at its own step count
+(hy_batch_set_stops) and runs as long as its longest member.  A group runs as one batch (hy_batch_*): each
 kernel of a step is launched once for all its models (grid z = model), so a
 step of B small runs costs the launches of one.  A batch is one slot; its
 models' device bytes are charged to the budget as a whole.  A group the
@@ -176,10 +177,18 @@ class _Slot:
             m.close()
 
 
+def steps_bucket(nsteps: int) -> int:
+    """Runs whose step counts share a bucket (within a factor of two) share a
+    batch: a batch runs as long as its longest member, and a shorter member is
+    reported when the batch ends."""
+    return max(0, int(math.ceil(math.log2(max(nsteps, 1) / 500.0))))
+
+
 def batch_key(req: Request) -> tuple:
-    """Requests with equal keys run the same kernel sequence."""
+    """Requests with equal keys run the same kernel sequence (a batch stops each
+    run at its own step count)."""
     p = req.p
-    return (p.nx, p.ny, p.nz, p.kind, p.closure, p.tau_model, p.vext, p.dt <= 0.0, req.nsteps,
+    return (p.nx, p.ny, p.nz, p.kind, p.closure, p.tau_model, p.vext, p.dt <= 0.0, steps_bucket(req.nsteps),
             tuple(req.mode), req.kind)
 
 
@@ -206,8 +215,6 @@ class Server:
     def _admit(self, reqs, now: float) -> list:
         """Start `reqs` (one batch key) as one batch, or each as a captured run."""
         req = reqs[0]
-        launches = -(-req.nsteps // self.chunk)
-        steps = launches * self.chunk
         dt_every = DT_EVERY if req.p.dt <= 0.0 else 0
         models = [make_model(r, self.precision, self.cache) for r in reqs]
         # each run is charged the larger of the estimate it was admitted on and its actual bytes
@@ -217,25 +224,33 @@ class Server:
         parts = {}
         for r, m, c in zip(reqs, models, charge):
             parts.setdefault(m.kept(), []).append((r, m, c))
+        def rounded(r):
+            return -(-r.nsteps // self.chunk) * self.chunk
+
         out = []
         for part in parts.values():
             if len(part) > 1:
                 from .model import Batch
                 from . import abi
                 ms = [m for _, m, _ in part]
+                steps = max(rounded(r) for r, _, _ in part)
                 try:
                     b = Batch(ms, self.chunk, dt_every=dt_every, probe=req.mode, ring=steps)
                 except abi.HydftError as e:
                     if e.code != abi.HY_ERR_INVALID:
                         raise
                 else:
-                    b.launch(launches)
+                    stops = [rounded(r) for r, _, _ in part]
+                    if min(stops) < steps:
+                        b.set_stops(stops)
+                    b.launch(steps // self.chunk)
                     self.batches_launched += 1
                     out.append(_Slot([r for r, _, _ in part], ms, now, sum(c for _, _, c in part), steps, b))
                     continue
             for r, m, c in part:
+                steps = rounded(r)
                 m.graph_build(self.chunk, dt_every=dt_every, probe=r.mode, ring=steps)
-                m.graph_launch(launches)
+                m.graph_launch(steps // self.chunk)
                 self.batches_launched += 1
                 out.append(_Slot([r], [m], now, c, steps))
         return out
@@ -295,7 +310,7 @@ class Server:
                     m.check()
                     rows = s.rows(i)[: req.nsteps]
                     done = time.perf_counter() - t0
-                    results.append(Result(req.rid, req.kind, req.arrival, s.start, done, s.steps, rows,
+                    results.append(Result(req.rid, req.kind, req.arrival, s.start, done, len(rows), rows,
                                           analyse(req, rows, linear_for_model(m))))
                 active.remove(s)
                 self._retired.append(s)

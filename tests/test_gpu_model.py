@@ -28,6 +28,10 @@ CASES = {
     "elec3d": (dict(kind="electron", gamma=1.0, rs=1.86, nx=16, ny=16, nz=16, lx=20.0, ly=20.0, lz=20.0,
                     eta=0.1, dt=0.01), 3),
     "elec1d": (dict(kind="electron", gamma=1.0, rs=1.86, nx=64, ny=1, nz=1, lx=20.0, eta=0.1, dt=0.01), 5),
+    # electrons with the Maxwell closure: the relaxation path of the x-line
+    # kernel (no functional there) with the gradient term in the model
+    "elec3d_maxwell": (dict(kind="electron", gamma=1.0, rs=1.86, nx=16, ny=16, nz=12, lx=20.0, ly=20.0, lz=15.0,
+                            eta=0.1, closure="maxwell", tau_model="constant", tau=0.5, dt=0.01), 3),
     "ion1d_driven": (dict(nx=32, lx=20.0, gamma=2.0, kappa=0.1, eta=0.3, vext="driven",
                           vext_amplitude=1e-3, vext_mode=(2, 0, 0), vext_omega=0.4, vext_ramp=0.05,
                           dt=0.02), 5),
