@@ -127,6 +127,9 @@ _SIGS = {
     "hy_batch_destroy": [_P],
     "hy_model_probe_ring": [_P, _I, _P],
     "hy_model_kept": [_P, _PI],
+    "hy_plan_tuning": [_I],
+    "hy_plan_reset": [],
+    "hy_plan_list": [_I, _P],
     "hy_fun_create": [_P, _I, _P, C.POINTER(_P)],
     "hy_fun_destroy": [_P],
     "hy_fun_nterms": [_P, _PI],
@@ -166,7 +169,7 @@ class _Lib:
 
     def call(self, name, *args):
         rc = getattr(self.lib, name)(*args)
-        if name == "hy_model_profile_read" and rc > 0:
+        if name in ("hy_model_profile_read", "hy_plan_list") and rc > 0:
             return rc
         if rc != HY_OK:
             raise HydftError(rc, self.lib.hy_last_error().decode())

@@ -5,7 +5,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 #include "hy_internal.h"
 #include "hy_regfft.h"
@@ -238,22 +240,33 @@ __device__ inline cpx<T> load_any(const void* p, size_t i, int is64) {
   return cvt<T>(gload_c<float>(p, i));
 }
 
-// Block shape of the pencil kernels: NA contiguous columns (FP32: 192 bytes
-// of one row for columns of 64 and 128 points, 128 bytes for columns of at
-// most 48, 256 bytes otherwise; 128 bytes in FP64) times TPC threads per column; columns
-// of at most 16 points take one thread each, 64 per block.
+// Block shape of the pencil kernels: NA contiguous columns times TPC threads
+// per column (the column plan of hy_regfft.h: columns of at most 16 points take
+// one thread each).  NA is a template parameter: the plan autotuner
+// (pencil_plan below) times the candidates of each pass shape once and keeps
+// the fastest.  The default (the choice without timing) takes 64 columns per
+// block for one-thread columns; FP32 takes 16 for columns of at most 48 points,
+// 24 at 64 and 128 points and 32 otherwise; FP64 takes 8.
 template <class T, int L>
+__host__ __device__ constexpr int pencil_tpc() { return col_plan(L, (int)sizeof(cpx<T>)).TPC; }
+template <class T, int L>
+__host__ __device__ constexpr int pencil_na_default() {
+  return pencil_tpc<T, L>() == 1 ? 64
+         : (sizeof(T) == 4 ? (L <= 48 ? 16 : (L == 64 || L == 128 ? 24 : 32)) : 8);
+}
+// a candidate: at most 1024 threads and 64 KiB of LDS per workgroup
+template <class T, int L, int NA>
+__host__ __device__ constexpr bool pencil_na_ok() {
+  return (pencil_tpc<T, L>() == 1 ? (NA == 32 || NA == 64 || NA == 128)
+                                  : (NA == 8 || NA == 16 || NA == 24 || NA == 32)) &&
+         NA * pencil_tpc<T, L>() <= 1024 &&
+         (pencil_tpc<T, L>() == 1 || (size_t)L * NA * sizeof(cpx<T>) <= 65536);
+}
+template <class T, int L, int NA_>
 struct PencilShape {
   static constexpr ColPlan P = col_plan(L, (int)sizeof(cpx<T>));
   static constexpr int L1 = P.L1, L2 = P.L2, TPC = P.TPC;
-  // FP32 columns of <= 48 points take 16 per block: small grids keep few x
-  // columns (11 at 32^3), and a narrower tile idles fewer lanes (measured
-  // 8 % faster batched steps at 32^3 and 48^3; slower at 64^3 and 256^3).
-  // 64- and 128-point columns take 24: the 22 kept x columns of a 64^3 grid
-  // fill one tile (2 % faster batched 64^3 steps than 32; 8 and 16 were
-  // slower), the 43 of a 128^3 grid two (3.5 % faster 128^3 Maxwell steps).
-  static constexpr int NA =
-      TPC == 1 ? 64 : (sizeof(T) == 4 ? (L <= 48 ? 16 : (L == 64 || L == 128 ? 24 : 32)) : 8);
+  static constexpr int NA = NA_;
   static constexpr int NT = NA * TPC;
   static constexpr int J1 = L2 / TPC;  // step-1 DFTs (length L1) per thread
   static constexpr int J2 = L1 / TPC;  // step-2 DFTs (length L2) per thread
@@ -271,10 +284,10 @@ struct PencilK {
   const cpx<T>* tw;
 };
 
-template <class T, int L>
+template <class T, int L, int NA_>
 __device__ __attribute__((always_inline)) inline void pencil_inv_body(const PencilArgs& a, const cpx<T>* __restrict__ twg) {
   if (gate_closed(a.halt)) return;  // uniform over the workgroup
-  using S = PencilShape<T, L>;
+  using S = PencilShape<T, L, NA_>;
   constexpr int L1 = S::L1, L2 = S::L2, TPC = S::TPC, NA = S::NA;
   const int al = threadIdx.x % NA, g = threadIdx.x / NA;
   // 1-D grid of ntile * B * nf workgroups in XCD-aware order
@@ -340,10 +353,10 @@ __device__ __attribute__((always_inline)) inline void pencil_inv_body(const Penc
 }
 
 // Forward pass: in (A, L, B) -> out (A, K, B) kept entries, scaled by 1/L.
-template <class T, int L>
+template <class T, int L, int NA_>
 __device__ __attribute__((always_inline)) inline void pencil_fwd_body(const PencilArgs& a, const cpx<T>* __restrict__ twg) {
   if (gate_closed(a.halt)) return;  // uniform over the workgroup
-  using S = PencilShape<T, L>;
+  using S = PencilShape<T, L, NA_>;
   constexpr int L1 = S::L1, L2 = S::L2, TPC = S::TPC, NA = S::NA;
   const int al = threadIdx.x % NA, g = threadIdx.x / NA;
   // 1-D grid of ntile * B * nf workgroups in XCD-aware order
@@ -402,44 +415,231 @@ __device__ __attribute__((always_inline)) inline void pencil_fwd_body(const Penc
   }
 }
 
-template <class T, int L>
-__global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_inv(PencilK<T> k) {
-  pencil_inv_body<T, L>(k.a, k.tw);
+template <class T, int L, int NA>
+__global__ __launch_bounds__((PencilShape<T, L, NA>::NT)) void k_pencil_inv(PencilK<T> k) {
+  pencil_inv_body<T, L, NA>(k.a, k.tw);
 }
-template <class T, int L>
-__global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_inv_b(const PencilK<T>* __restrict__ ks) {
-  pencil_inv_body<T, L>(ks[blockIdx.z].a, ks[blockIdx.z].tw);
+template <class T, int L, int NA>
+__global__ __launch_bounds__((PencilShape<T, L, NA>::NT)) void k_pencil_inv_b(const PencilK<T>* __restrict__ ks) {
+  pencil_inv_body<T, L, NA>(ks[blockIdx.z].a, ks[blockIdx.z].tw);
 }
-template <class T, int L>
-__global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_fwd(PencilK<T> k) {
-  pencil_fwd_body<T, L>(k.a, k.tw);
+template <class T, int L, int NA>
+__global__ __launch_bounds__((PencilShape<T, L, NA>::NT)) void k_pencil_fwd(PencilK<T> k) {
+  pencil_fwd_body<T, L, NA>(k.a, k.tw);
 }
-template <class T, int L>
-__global__ __launch_bounds__((PencilShape<T, L>::NT)) void k_pencil_fwd_b(const PencilK<T>* __restrict__ ks) {
-  pencil_fwd_body<T, L>(ks[blockIdx.z].a, ks[blockIdx.z].tw);
+template <class T, int L, int NA>
+__global__ __launch_bounds__((PencilShape<T, L, NA>::NT)) void k_pencil_fwd_b(const PencilK<T>* __restrict__ ks) {
+  pencil_fwd_body<T, L, NA>(ks[blockIdx.z].a, ks[blockIdx.z].tw);
 }
 
+// The single and batched kernels of a pencil plan, and its launch shape.
+struct PencilPlanFns {
+  const void* single = nullptr;
+  const void* batched = nullptr;
+  unsigned nt = 0;
+};
+template <class T, int L, int NA>
+static PencilPlanFns pencil_fns_na(bool inverse) {
+  PencilPlanFns p;
+  if constexpr (pencil_na_ok<T, L, NA>()) {
+    p.single = inverse ? reinterpret_cast<const void*>(&k_pencil_inv<T, L, NA>)
+                       : reinterpret_cast<const void*>(&k_pencil_fwd<T, L, NA>);
+    p.batched = inverse ? reinterpret_cast<const void*>(&k_pencil_inv_b<T, L, NA>)
+                        : reinterpret_cast<const void*>(&k_pencil_fwd_b<T, L, NA>);
+    p.nt = (unsigned)PencilShape<T, L, NA>::NT;
+  }
+  return p;
+}
 template <class T, int L>
-static int launch_pencil(const PencilArgs& a, bool inverse, int nf, const cpx<T>* tw, hipStream_t s) {
-  using S = PencilShape<T, L>;
-  const dim3 grid((unsigned)((a.A + S::NA - 1) / S::NA) * (unsigned)a.B * (unsigned)nf);
+static PencilPlanFns pencil_fns_l(bool inverse, int na) {
+  switch (na) {
+    case 8: return pencil_fns_na<T, L, 8>(inverse);
+    case 16: return pencil_fns_na<T, L, 16>(inverse);
+    case 24: return pencil_fns_na<T, L, 24>(inverse);
+    case 32: return pencil_fns_na<T, L, 32>(inverse);
+    case 64: return pencil_fns_na<T, L, 64>(inverse);
+    case 128: return pencil_fns_na<T, L, 128>(inverse);
+  }
+  return PencilPlanFns{};
+}
+template <class T>
+static PencilPlanFns pencil_fns(int L, bool inverse, int na) {
+  switch (L) {
+#define HY_PL(LL) case LL: return pencil_fns_l<T, LL>(inverse, na);
+    HY_PL(4) HY_PL(8) HY_PL(12) HY_PL(16) HY_PL(24) HY_PL(32) HY_PL(48) HY_PL(64) HY_PL(96)
+    HY_PL(128) HY_PL(256)
+#undef HY_PL
+  }
+  return PencilPlanFns{};
+}
+template <class T>
+static int pencil_default_na(int L) {
+  switch (L) {
+#define HY_PL(LL) case LL: return pencil_na_default<T, LL>();
+    HY_PL(4) HY_PL(8) HY_PL(12) HY_PL(16) HY_PL(24) HY_PL(32) HY_PL(48) HY_PL(64) HY_PL(96)
+    HY_PL(128) HY_PL(256)
+#undef HY_PL
+  }
+  return 0;
+}
+static const int kNaCandidates[] = {8, 16, 24, 32, 64, 128};
+
+// ------------------------------------------------------------ plan autotuner
+// The column count per workgroup of each pencil pass shape is chosen by
+// timing the candidates once, on the pass's own buffers, the first time the
+// shape runs outside a graph capture or a batch recording (FFTW_MEASURE's role
+// in the reference, fft.f90:46-47), and kept for the process.  Only
+// out-of-place passes are timed (a candidate's run writes the same output as
+// the pass itself); in-place passes and passes inside a capture take the
+// shape's known choice or the default.  HYDFT_TUNE=0 keeps the defaults.  The
+// arithmetic of a column does not depend on the choice, so results are
+// bitwise the same under every plan.
+struct PlanKey {
+  int f64, L, inv, A, B, nf, z;
+  bool operator<(const PlanKey& o) const {
+    return std::tie(f64, L, inv, A, B, nf, z) < std::tie(o.f64, o.L, o.inv, o.A, o.B, o.nf, o.z);
+  }
+};
+static std::mutex g_plan_mutex;
+static std::map<PlanKey, int> g_plans;
+static int g_tune = -1;  // -1: from HYDFT_TUNE (default on)
+static bool tuning_enabled() {
+  std::lock_guard<std::mutex> lock(g_plan_mutex);
+  if (g_tune < 0) {
+    const char* e = std::getenv("HYDFT_TUNE");
+    g_tune = (e && e[0] == '0') ? 0 : 1;
+  }
+  return g_tune == 1;
+}
+static int plan_lookup(const PlanKey& k) {
+  std::lock_guard<std::mutex> lock(g_plan_mutex);
+  auto it = g_plans.find(k);
+  return it == g_plans.end() ? 0 : it->second;
+}
+static void plan_store(const PlanKey& k, int na) {
+  std::lock_guard<std::mutex> lock(g_plan_mutex);
+  g_plans[k] = na;
+}
+
+// Launches of one candidate: `reps` runs, the time of the fastest (ms).
+static int time_launch(const void* fn, dim3 grid, dim3 block, void** args, hipStream_t s, int reps, float* best) {
+  hipEvent_t e0, e1;
+  HY_HIP(hipEventCreate(&e0));
+  HY_HIP(hipEventCreate(&e1));
+  *best = 1e30f;
+  int rc = HY_OK;
+  for (int r = 0; r < reps && rc == HY_OK; ++r) {
+    if (hipEventRecord(e0, s) != hipSuccess || hipLaunchKernel(fn, grid, block, args, 0, s) != hipSuccess ||
+        hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+      (void)hipGetLastError();
+      rc = set_error(HY_ERR_HIP, "pencil plan timing failed");
+      break;
+    }
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (r > 0 || reps == 1) *best = std::min(*best, ms);  // the first run warms the caches
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+template <class T>
+static unsigned pencil_grid_x(const PencilArgs& a, int nf, int na) {
+  return (unsigned)((a.A + na - 1) / na) * (unsigned)a.B * (unsigned)nf;
+}
+
+// The plan of a single (not batched) pass: the known one, or one measured now.
+template <class T>
+static int choose_na(const PencilArgs& a, bool inverse, int nf, const cpx<T>* tw, hipStream_t s) {
+  const PlanKey key{sizeof(T) == 8 ? 1 : 0, a.L, inverse ? 1 : 0, a.A, a.B, nf, 1};
+  if (int na = plan_lookup(key)) return na;
+  const int na0 = pencil_default_na<T>(a.L);
+  bool can = tuning_enabled() && !recorder();
+  if (can) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) can = false;
+    for (int i = 0; i < nf && can; ++i) can = a.f[i].in != a.f[i].out;  // in-place passes are not repeated
+  }
+  if (!can) return na0;
   const PencilK<T> k{a, tw};
-  if (inverse)
-    return launch(reinterpret_cast<const void*>(&k_pencil_inv<T, L>), reinterpret_cast<const void*>(&k_pencil_inv_b<T, L>),
-                  grid, dim3(S::NT), s, k);
-  return launch(reinterpret_cast<const void*>(&k_pencil_fwd<T, L>), reinterpret_cast<const void*>(&k_pencil_fwd_b<T, L>),
-                grid, dim3(S::NT), s, k);
+  void* args[] = {const_cast<PencilK<T>*>(&k)};
+  int best_na = na0;
+  float best = 1e30f;
+  for (int na : kNaCandidates) {
+    const PencilPlanFns p = pencil_fns<T>(a.L, inverse, na);
+    if (!p.single) continue;
+    float ms;
+    HY_TRY(time_launch(p.single, dim3(pencil_grid_x<T>(a, nf, na)), dim3(p.nt), args, s, 3, &ms));
+    if (ms < best * 0.98f || (na == na0 && ms <= best * 1.02f)) {  // ties keep the default
+      best = ms;
+      best_na = na;
+    }
+  }
+  plan_store(key, best_na);
+  return best_na;
+}
+
+// The plan of a batched pass (a batch of `nz` runs, grid z = run): the known
+// one for this batch size, or one measured now with the batch's own packed
+// arguments (the candidates rewrite the runs' transform slots, which the next
+// launch recomputes from the state).
+template <class T>
+static int batch_plan_t(const std::array<int, 8>& pk, int nz, void* d_args, hipStream_t s, const void** fn,
+                        dim3* grid, dim3* block) {
+  const int L = pk[2], inv = pk[3], A = pk[4], B = pk[5], nf = pk[6];
+  const PlanKey key{pk[1], L, inv, A, B, nf, nz};
+  int na = plan_lookup(key);
+  PencilArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.A = A;
+  a.B = B;
+  if (!na) {
+    na = pk[7];
+    if (tuning_enabled()) {
+      void* args[] = {&d_args};
+      float best = 1e30f;
+      for (int c : kNaCandidates) {
+        const PencilPlanFns p = pencil_fns<T>(L, inv != 0, c);
+        if (!p.batched) continue;
+        float ms;
+        HY_TRY(time_launch(p.batched, dim3(pencil_grid_x<T>(a, nf, c), 1, (unsigned)nz), dim3(p.nt), args, s, 3, &ms));
+        if (ms < best * 0.98f || (c == pk[7] && ms <= best * 1.02f)) {
+          best = ms;
+          na = c;
+        }
+      }
+      plan_store(key, na);
+    }
+  }
+  const PencilPlanFns p = pencil_fns<T>(L, inv != 0, na);
+  if (!p.batched) return set_error(HY_ERR_INVALID, "pencil plan: no kernel for the chosen shape");
+  *fn = p.batched;
+  *grid = dim3(pencil_grid_x<T>(a, nf, na), 1, (unsigned)nz);
+  *block = dim3(p.nt);
+  return HY_OK;
+}
+int pencil_batch_plan(const std::array<int, 8>& pk, int nz, void* d_args, hipStream_t s, const void** fn,
+                      dim3* grid, dim3* block) {
+  return pk[1] ? batch_plan_t<double>(pk, nz, d_args, s, fn, grid, block)
+               : batch_plan_t<float>(pk, nz, d_args, s, fn, grid, block);
 }
 
 template <class T>
 static int dispatch_pencil(const PencilArgs& a, bool inverse, int nf, const cpx<T>* tw, hipStream_t s) {
-  switch (a.L) {
-#define HY_PL(LL) case LL: HY_TRY((launch_pencil<T, LL>(a, inverse, nf, tw, s))); break;
-    HY_PL(4) HY_PL(8) HY_PL(12) HY_PL(16) HY_PL(24) HY_PL(32) HY_PL(48) HY_PL(64) HY_PL(96)
-    HY_PL(128) HY_PL(256)
-#undef HY_PL
-    default: return set_error(HY_ERR_UNSUPPORTED, "pencil: unsupported length");
+  const int na = choose_na<T>(a, inverse, nf, tw, s);
+  const PencilPlanFns p = pencil_fns<T>(a.L, inverse, na);
+  if (!p.single) return set_error(HY_ERR_UNSUPPORTED, "pencil: unsupported length");
+  const PencilK<T> k{a, tw};
+  if (Recorder* r = recorder()) {
+    LaunchRec it{p.batched, dim3(pencil_grid_x<T>(a, nf, na)), dim3(p.nt), {}};
+    it.args.resize(sizeof(k));
+    std::memcpy(it.args.data(), &k, sizeof(k));
+    it.pencil = {1, sizeof(T) == 8 ? 1 : 0, a.L, inverse ? 1 : 0, a.A, a.B, nf, na};
+    r->items.push_back(std::move(it));
+    return HY_OK;
   }
+  HY_TRY(launch(p.single, p.batched, dim3(pencil_grid_x<T>(a, nf, na)), dim3(p.nt), s, k));
   HY_HIP(hipGetLastError());
   HY_TRACE(inverse ? "pencil_inverse" : "pencil_forward", s);
   return HY_OK;
@@ -1050,6 +1250,33 @@ int hy_stress_div(hy_grid gh, const double* d_pi6, double* d_f3, void* stream) {
     HY_TRY(fft_c2r_full_inplace(g, s[0], dst, st));
   }
   return HY_OK;
+}
+
+// ---- pencil plans (the autotuner's choices)
+int hy_plan_tuning(int on) {
+  std::lock_guard<std::mutex> lock(g_plan_mutex);
+  g_tune = on ? 1 : 0;
+  return HY_OK;
+}
+
+int hy_plan_reset(void) {
+  std::lock_guard<std::mutex> lock(g_plan_mutex);
+  g_plans.clear();
+  return HY_OK;
+}
+
+int hy_plan_list(int max, int* out8) {
+  std::lock_guard<std::mutex> lock(g_plan_mutex);
+  int n = 0;
+  for (const auto& kv : g_plans) {
+    if (out8 && n < max) {
+      const PlanKey& k = kv.first;
+      const int v[8] = {k.f64, k.L, k.inv, k.A, k.B, k.nf, k.z, kv.second};
+      std::memcpy(out8 + 8 * n, v, sizeof(v));
+    }
+    ++n;
+  }
+  return n;
 }
 
 }  // extern "C"

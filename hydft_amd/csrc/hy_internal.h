@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "../../include/hydft.h"
@@ -49,6 +50,9 @@ struct LaunchRec {
   bool xline = false;
   const void* xblock = nullptr;
   const void* xtw = nullptr;
+  // pencil passes: {1, f64, L, inverse, A, B, nf, columns per workgroup}
+  // (the batch retunes the plan for its size); pencil[0] == 0 otherwise
+  std::array<int, 8> pencil{};
 };
 struct Recorder {
   std::vector<LaunchRec> items;
@@ -287,6 +291,12 @@ constexpr int MAXPF = 24;
 // when set (a run that reached its step limit in a batch); null: never.
 int pencil_pass(const Grid& g, int axis, bool inverse, bool trunc, int A, int B,
                 const PencilField* f, int nf, hipStream_t s, const int* halt = nullptr);
+
+// The plan of a recorded pencil pass (LaunchRec::pencil) in a batch of nz
+// runs whose packed arguments are at d_args: the batched kernel, grid and block
+// to launch (measured once per shape and batch size, then kept).
+int pencil_batch_plan(const std::array<int, 8>& pencil, int nz, void* d_args, hipStream_t s, const void** fn,
+                      dim3* grid, dim3* block);
 
 // A device gate flag, read through the global address space.
 __device__ inline bool gate_closed(const int* h) {

@@ -2649,6 +2649,14 @@ int hy_batch_create(const hy_model* ms, int n, int nsteps, int dt_every, const i
     // the twiddle table of a line length is the same in every model of the batch
     b->items.push_back({it.fn_batched, dim3(it.grid.x, it.grid.y, (unsigned)n), it.block,
                         (char*)b->d_all + off[p], it.xline, it.xtw});
+    if (it.pencil[0]) {  // pencil passes: the plan for this batch size
+      auto& bi = b->items.back();
+      const int rc = pencil_batch_plan(it.pencil, n, bi.d_args, b->st, &bi.fn, &bi.grid, &bi.block);
+      if (rc != HY_OK) {
+        batch_free(b);
+        return rc;
+      }
+    }
   }
   for (hy_model h : b->models) ++h->m.nbatch;
   b->holds = true;

@@ -67,6 +67,20 @@ int hy_grid_mask(hy_grid g, double* h_mask);
 int hy_fft_r2c(hy_grid g, const double* d_u, double* d_uhat, void* stream);
 int hy_fft_c2r(hy_grid g, const double* d_uhat, double* d_u, void* stream);
 
+/* ---- pencil plans ---------------------------------------------------------
+   The y/z transform passes run with a column count per workgroup chosen by
+   timing the candidates once per pass shape (precision, length, direction,
+   column count, outer count, fields, batch size) on its first run outside a
+   graph capture or a batch recording, and kept for the process: the role of
+   FFTW_MEASURE in fft_t%init (fft.f90:46-47).  Results do not depend on the
+   plan.  hy_plan_tuning(0) keeps the untimed defaults (as HYDFT_TUNE=0 does);
+   hy_plan_reset forgets the choices; hy_plan_list writes up to `max` entries
+   {fp64, L, inverse, A, B, nfields, batch, columns per workgroup} and returns
+   the number of entries. */
+int hy_plan_tuning(int on);
+int hy_plan_reset(void);
+int hy_plan_list(int max, int* out8);
+
 /* ---- spec_t (src/core/spectral.f90:67-188), FP64 device buffers --------- */
 int hy_spec_grad_hat(hy_grid g, const double* d_uhat, int idir, double* d_du, void* stream);
 int hy_spec_grad(hy_grid g, const double* d_u, double* d_du3, void* stream);
