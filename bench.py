@@ -91,6 +91,17 @@ def cpu_baseline(p, st, n, mom):
             "sample": f"one RK3 stage (of 3) of the FP64 oracle at {p.nx}^3, {el:.1f} s, scaled to steps/s"}
 
 
+def pencil_plans():
+    """The autotuner's choices so far: {fp64, L, inverse, A, B, nfields, batch,
+    columns per workgroup} per pass shape (hy_plan_list)."""
+    from hydft_amd import abi
+    lib = abi.lib().lib
+    n = lib.hy_plan_list(0, None)
+    buf = (C.c_int * (8 * max(n, 1)))()
+    lib.hy_plan_list(n, buf)
+    return [list(buf[8 * i:8 * i + 8]) for i in range(n)]
+
+
 def copy_probe(mib):
     """A plain copy kernel of known size: reads and writes mib MiB once per launch, 3 launches."""
     import torch
@@ -489,6 +500,7 @@ def main():
                           "GBps": r["bytes"] / (r["total_ms"] * 1e-3) / 1e9} for r in prof],
              "step_hbm_GBps": tot_bytes / (tot_ms * 1e-3) / 1e9,
              "bytes_per_step": tot_bytes / args.steps}
+    extra["pencil_plans"] = pencil_plans()
     if traffic:
         extra["traffic_per_launch"] = {r["kernel"]: kernel_traffic(traffic, r["kernel"], args.precision)
                                        for r in prof}

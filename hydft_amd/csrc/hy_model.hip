@@ -469,7 +469,6 @@ struct PhysArgs {
   constexpr int PTS = X::PTS;                                                          \
   __shared__ typename X::Smem sm;                                                      \
   const PhysArgs& a = *pa;                                                             \
-  if (gate_closed(&a.ctl->halt)) return;                                               \
   const int lane = threadIdx.x;                                                        \
   const int lw = lane / X::TPC, g = lane % X::TPC;                                     \
   const size_t line = (size_t)xcd_work_index(blockIdx.x, gridDim.x) * X::LPW + lw;     \
@@ -696,6 +695,7 @@ __global__ __launch_bounds__(64) void k_phaseA(const PhysArgs* __restrict__ pa, 
 }
 template <class T, int NX, class X>
 __global__ __launch_bounds__(64) void k_phaseA_b(const PhysArgs* __restrict__ pas, const cpx<T>* __restrict__ tw) {
+  if (gate_closed(&pas[blockIdx.z].ctl->halt)) return;
   phaseA_body<T, NX, X>(pas + blockIdx.z, tw);
 }
 template <class T, int NX, class X, bool MX>
@@ -704,6 +704,7 @@ __global__ __launch_bounds__(64) void k_phaseB(const PhysArgs* __restrict__ pa, 
 }
 template <class T, int NX, class X, bool MX>
 __global__ __launch_bounds__(64) void k_phaseB_b(const PhysArgs* __restrict__ pas, const cpx<T>* __restrict__ tw) {
+  if (gate_closed(&pas[blockIdx.z].ctl->halt)) return;
   phaseB_body<T, NX, X, MX>(pas + blockIdx.z, tw);
 }
 template <class T, int NX, class X>
@@ -712,6 +713,7 @@ __global__ __launch_bounds__(64) void k_umax(const PhysArgs* __restrict__ pa, co
 }
 template <class T, int NX, class X>
 __global__ __launch_bounds__(64) void k_umax_b(const PhysArgs* __restrict__ pas, const cpx<T>* __restrict__ tw) {
+  if (gate_closed(&pas[blockIdx.z].ctl->halt)) return;
   umax_body<T, NX, X>(pas + blockIdx.z, tw);
 }
 
@@ -841,7 +843,6 @@ struct K1Args {
 
 template <class T>
 __device__ __attribute__((always_inline)) inline void spec1_body(const K1Args& a) {
-  if (gate_closed(&a.ctl->halt)) return;
   const size_t nt = a.s.nt;
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nt; e += (size_t)gridDim.x * blockDim.x) {
     c64 nh = gldc<double>(a.N, e);
@@ -872,7 +873,10 @@ __device__ __attribute__((always_inline)) inline void spec1_body(const K1Args& a
 template <class T>
 __global__ __launch_bounds__(256) void k_spec1(K1Args a) { spec1_body<T>(a); }
 template <class T>
-__global__ __launch_bounds__(256) void k_spec1_b(const K1Args* __restrict__ as) { spec1_body<T>(as[blockIdx.z]); }
+__global__ __launch_bounds__(256) void k_spec1_b(const K1Args* __restrict__ as) {
+  if (gate_closed(&as[blockIdx.z].ctl->halt)) return;  // a run past its step limit in a batch
+  spec1_body<T>(as[blockIdx.z]);
+}
 
 struct K2Args {
   SpecArgs s;
@@ -897,7 +901,6 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
   const size_t nt = a.s.nt;
   constexpr int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
   constexpr int idx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};  // stress.f90:92
-  if (gate_closed(&a.ctl->halt)) return;
   const double dt = a.ctl->dt;
   const double ra = c_rk_a[a.stage], rb = c_rk_b[a.stage];
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nt; e += (size_t)gridDim.x * blockDim.x) {
@@ -978,7 +981,10 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
 template <class T, int ND>
 __global__ __launch_bounds__(256) void k_spec2(K2Args a) { spec2_body<T, ND>(a); }
 template <class T, int ND>
-__global__ __launch_bounds__(256) void k_spec2_b(const K2Args* __restrict__ as) { spec2_body<T, ND>(as[blockIdx.z]); }
+__global__ __launch_bounds__(256) void k_spec2_b(const K2Args* __restrict__ as) {
+  if (gate_closed(&as[blockIdx.z].ctl->halt)) return;
+  spec2_body<T, ND>(as[blockIdx.z]);
+}
 
 // probe: spec_mode of n/n0 and mom_x (diagnostics.f90:70-77) after a step
 __global__ void k_probe(const Ctl* c, const c64* S, size_t nt, long long idx, double n0, double* out) {
@@ -1186,6 +1192,11 @@ static double pencil_bytes(const Model& m, const std::vector<PencilField>& f, si
   return b;
 }
 
+// The step-limit gate of a pass: only passes recorded for a batch take it
+// (a run's own launches and graphs have no step limit, and a gate read at
+// the start of every workgroup costs latency).
+static const int* batch_gate(const Model& m) { return recorder() ? &m.ctl->halt : nullptr; }
+
 // ---- inverse chains: truncated spectrum -> x-line slot (returns the XIn)
 struct InvSpec {
   const void* src;
@@ -1208,11 +1219,11 @@ static int inv_chain(Model& m, const std::vector<InvSpec>& fs, std::vector<XIn>&
     }
     if ((int)fs.size() > m.nzslots) return set_error(HY_ERR_INVALID, "internal: z slots");
     prof_begin(m, PC_ZINV, pencil_bytes(m, zf, (size_t)g.Kx * g.Ky, 1, g.nz, g.Kz));
-    HY_TRY(pencil_pass(g, 2, true, true, g.Kx * g.Ky, 1, zf.data(), (int)zf.size(), m.st, &m.ctl->halt));
+    HY_TRY(pencil_pass(g, 2, true, true, g.Kx * g.Ky, 1, zf.data(), (int)zf.size(), m.st, batch_gate(m)));
     prof_end(m);
     if (g.ny > 1) {
       prof_begin(m, PC_YINV, pencil_bytes(m, yf, g.Kx, g.nz, g.ny, g.Ky));
-      HY_TRY(pencil_pass(g, 1, true, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st, &m.ctl->halt));
+      HY_TRY(pencil_pass(g, 1, true, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st, batch_gate(m)));
       prof_end(m);
     }
     for (auto& f : fs) xin.push_back({xslot(m, f.xslot), 0, f.mx, 0.0});
@@ -1223,7 +1234,7 @@ static int inv_chain(Model& m, const std::vector<InvSpec>& fs, std::vector<XIn>&
     std::vector<PencilField> yf;
     for (auto& f : fs) yf.push_back({f.src, xslot(m, f.xslot), f.my, f.src64 ? 1 : ct64(m), f.k0off});
     prof_begin(m, PC_YINV, pencil_bytes(m, yf, g.Kx, 1, g.ny, g.Ky));
-    HY_TRY(pencil_pass(g, 1, true, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st, &m.ctl->halt));
+    HY_TRY(pencil_pass(g, 1, true, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st, batch_gate(m)));
     prof_end(m);
     for (auto& f : fs) xin.push_back({xslot(m, f.xslot), 0, f.mx, 0.0});
   } else {
@@ -1251,17 +1262,17 @@ static int fwd_chain(Model& m, const std::vector<std::pair<int, int>>& f /* (xsl
     if ((int)f.size() > m.nzslots) return set_error(HY_ERR_INVALID, "internal: z slots");
     if (g.ny > 1) {
       prof_begin(m, PC_YFWD, pencil_bytes(m, yf, g.Kx, g.nz, g.ny, g.Ky));
-      HY_TRY(pencil_pass(g, 1, false, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st, &m.ctl->halt));
+      HY_TRY(pencil_pass(g, 1, false, true, g.Kx, g.nz, yf.data(), (int)yf.size(), m.st, batch_gate(m)));
       prof_end(m);
     }
     prof_begin(m, PC_ZFWD, pencil_bytes(m, zf, (size_t)g.Kx * g.Ky, 1, g.nz, g.Kz));
-    HY_TRY(pencil_pass(g, 2, false, true, g.Kx * g.Ky, 1, zf.data(), (int)zf.size(), m.st, &m.ctl->halt));
+    HY_TRY(pencil_pass(g, 2, false, true, g.Kx * g.Ky, 1, zf.data(), (int)zf.size(), m.st, batch_gate(m)));
     prof_end(m);
   } else if (g.ny > 1) {
     std::vector<PencilField> yf;
     for (auto& p : f) yf.push_back({xslot(m, p.first), fslot(m, p.second), MUL_NONE, ct64(m), 0.0});
     prof_begin(m, PC_YFWD, pencil_bytes(m, yf, g.Kx, 1, g.ny, g.Ky));
-    HY_TRY(pencil_pass(g, 1, false, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st, &m.ctl->halt));
+    HY_TRY(pencil_pass(g, 1, false, true, g.Kx, 1, yf.data(), (int)yf.size(), m.st, batch_gate(m)));
     prof_end(m);
   }
   return HY_OK;
