@@ -127,8 +127,12 @@ __global__ __launch_bounds__(TTILE / tab_rows<NA>()) void k_dst_tab(const double
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     j[r] = blockIdx.x * TTILE + r * NT + threadIdx.x;
-    step[r] = j[r] + 1;  // < two, so one conditional subtraction keeps q in range
-    q[r] = (int)(((long long)(j[r] + 1) * (long long)(m0 + 1)) % two);
+    // rows past the last (their sums are discarded) stay on entry 0; for a
+    // live row step = j + 1 <= nr < two, so one conditional subtraction keeps
+    // q in [0, two)
+    const bool live = j[r] < nr;
+    step[r] = live ? j[r] + 1 : 0;
+    q[r] = live ? (int)(((long long)(j[r] + 1) * (long long)(m0 + 1)) % two) : 0;
   }
   double acc[R][NA];
 #pragma unroll
@@ -351,6 +355,10 @@ int hy_hnc_iterate(int nsolve, int nr, double dr, double dk, const double* h_n0,
       !(dr > 0.0) || !(dk > 0.0) || maxiter < 1 || !(mix > 0.0))
     return set_error(HY_ERR_INVALID, "hy_hnc_iterate: bad argument");
   if ((size_t)nr * nr > (size_t)1 << 31) return set_error(HY_ERR_INVALID, "hy_hnc_iterate: nr too large");
+  // one call at a time: a call with another nr replaces the cached DST-I table
+  // and matrix, which a concurrent call may still be using
+  static std::mutex call_mutex;
+  std::lock_guard<std::mutex> call_lock(call_mutex);
   const int expcap_i = 40, npicard = 5, nattempt = 4;
   const Dims d{nr, dr, dk, (double)expcap_i};
   const size_t vec = (size_t)nsolve * nr;

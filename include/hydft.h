@@ -296,8 +296,9 @@ int hy_model_kept(hy_model m, int* kept3);
    attempt and the last residual.  The call keeps the DST-I table of the
    last nr (16 (nr+1) bytes) on the device for the life of the process,
    and the 8 nr^2-byte DST-I matrix as well when nsolve > 192 or
-   nr > 8192.  No counterpart as one call in the reference (it solves one
-   state point at a time on the host). */
+   nr > 8192.  Calls from several host threads run one after another.  No
+   counterpart as one call in the reference (it solves one state point at a
+   time on the host). */
 int hy_hnc_iterate(int nsolve, int nr, double dr, double dk, const double* h_n0, const double* h_bvs,
                    const double* h_bvlk, const double* h_gs0, double tol, double mix, int maxiter, double* h_gs,
                    int* h_status, int* h_niter, double* h_resid);

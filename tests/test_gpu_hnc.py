@@ -82,3 +82,18 @@ def test_table_and_gemm_dst_paths_agree(gpu):
     print("max rel c(k) difference", d)
     # both stop at max|gamma_new - gamma_s| < 1e-9 (summation order differs)
     assert d < 1e-8
+
+
+def test_device_hnc_small_radial_grid(gpu):
+    """nr = 200: the table DST passes' workgroups of 1024 rows then hold more
+    rows past the last than the table has entries; those rows stay on entry 0
+    and the live rows match the host solver."""
+    pts = _points()[:3]
+    nr, rmax = 200, 20.0
+    tol, mix, maxiter = 1e-9, 0.2, 20000
+    dev = hnc.hnc_solve_many_device([p[0] for p in pts], [p[1] for p in pts], nr, rmax, tol, mix, maxiter)
+    for i, p in enumerate(pts):
+        b = hnc.hnc_solve(p[0], p[1], nr, rmax, tol, mix, maxiter)
+        a = dev[i]
+        assert a.converged and b.converged, i
+        assert np.max(np.abs(a.ck - b.ck)) <= 1e-6 * np.max(np.abs(b.ck)), i
