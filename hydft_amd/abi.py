@@ -122,6 +122,11 @@ _SIGS = {
     "hy_batch_create": [_P, _I, _I, _I, _PI, _I, C.POINTER(_P)],
     "hy_batch_launch": [_P, _I],
     "hy_batch_set_stops": [_P, _P],
+    "hy_group_create": [_P, _I, C.POINTER(_P)],
+    "hy_group_launch": [_P, _I],
+    "hy_group_idle": [_P, _PI],
+    "hy_group_sync": [_P],
+    "hy_group_destroy": [_P],
     "hy_batch_idle": [_P, _PI],
     "hy_batch_sync": [_P],
     "hy_batch_destroy": [_P],

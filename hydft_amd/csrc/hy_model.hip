@@ -2676,21 +2676,135 @@ int hy_batch_create(const hy_model* ms, int n, int nsteps, int dt_every, const i
   return HY_OK;
 }
 
+// One pass of a batch's recorded launches on stream st.
+static int batch_enqueue(hy_batch b, hipStream_t st) {
+  for (auto& it : b->items) {
+    hipError_t e;
+    if (it.xline) {
+      void* args[] = {&it.d_args, const_cast<void**>(&it.tw)};
+      e = hipLaunchKernel(it.fn, it.grid, it.block, args, 0, st);
+    } else {
+      void* args[] = {&it.d_args};
+      e = hipLaunchKernel(it.fn, it.grid, it.block, args, 0, st);
+    }
+    if (e != hipSuccess) return set_error(HY_ERR_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+  }
+  return HY_OK;
+}
+
 int hy_batch_launch(hy_batch b, int nlaunch) {
   if (!b || nlaunch < 0) return set_error(HY_ERR_INVALID, "hy_batch_launch: bad argument");
   for (hy_model h : b->models) HY_HIP(hipStreamSynchronize(h->m.st));  // the models' own queued work first
-  for (int l = 0; l < nlaunch; ++l)
-    for (auto& it : b->items) {
-      hipError_t e;
-      if (it.xline) {
-        void* args[] = {&it.d_args, const_cast<void**>(&it.tw)};
-        e = hipLaunchKernel(it.fn, it.grid, it.block, args, 0, b->st);
-      } else {
-        void* args[] = {&it.d_args};
-        e = hipLaunchKernel(it.fn, it.grid, it.block, args, 0, b->st);
-      }
-      if (e != hipSuccess) return set_error(HY_ERR_HIP, std::string("hy_batch_launch: ") + hipGetErrorString(e));
+  for (int l = 0; l < nlaunch; ++l) HY_TRY(batch_enqueue(b, b->st));
+  return HY_OK;
+}
+
+// ---- groups of batches: runs on uneven grids advanced by one graph launch
+struct hy_group_s {
+  std::vector<hy_batch> batches;
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<hipEvent_t> events;
+  hipEvent_t done = nullptr;  // recorded after each hy_group_launch
+};
+
+static void group_free(hy_group_s* g) {
+  if (!g) return;
+  if (g->st) (void)hipStreamSynchronize(g->st);
+  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  for (auto e : g->events) (void)hipEventDestroy(e);
+  if (g->done) (void)hipEventDestroy(g->done);
+  if (g->st) (void)hipStreamDestroy(g->st);
+  delete g;
+}
+
+int hy_group_create(const hy_batch* batches, int n, hy_group* out) {
+  if (!batches || n < 1 || !out) return set_error(HY_ERR_INVALID, "hy_group_create: bad argument");
+  for (int i = 0; i < n; ++i) {
+    if (!batches[i]) return set_error(HY_ERR_INVALID, "hy_group_create: null batch");
+    for (int j = 0; j < i; ++j) {
+      if (batches[i] == batches[j]) return set_error(HY_ERR_INVALID, "hy_group_create: a batch appears twice");
+      // the branches run concurrently: a model in two of them would race
+      for (hy_model a : batches[i]->models)
+        for (hy_model c : batches[j]->models)
+          if (a == c) return set_error(HY_ERR_INVALID, "hy_group_create: a model is in two batches");
     }
+  }
+  auto* g = new hy_group_s();
+  g->batches.assign(batches, batches + n);
+  auto fail = [&](int rc) {
+    group_free(g);
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_error(HY_ERR_HIP, "hy_group_create: stream"));
+  for (hy_batch b : g->batches)
+    if (hipStreamSynchronize(b->st) != hipSuccess) return fail(set_error(HY_ERR_HIP, "hy_group_create: sync"));
+  if (hipEventCreateWithFlags(&g->done, hipEventDisableTiming) != hipSuccess)
+    return fail(set_error(HY_ERR_HIP, "hy_group_create: event"));
+  g->events.resize(1 + n, nullptr);
+  for (auto& e : g->events)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return fail(set_error(HY_ERR_HIP, "hy_group_create: event"));
+  // capture: fork one branch per batch (its own stream joins the capture
+  // through an event), each branch one pass of the batch's launches, join
+  if (hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return fail(set_error(HY_ERR_HIP, "hy_group_create: begin capture"));
+  int rc = HY_OK;
+  if (hipEventRecord(g->events[0], g->st) != hipSuccess) rc = set_error(HY_ERR_HIP, "hy_group_create: fork");
+  for (int i = 0; i < n && rc == HY_OK; ++i) {
+    hy_batch b = g->batches[i];
+    if (hipStreamWaitEvent(b->st, g->events[0], 0) != hipSuccess) { rc = set_error(HY_ERR_HIP, "hy_group_create: branch"); break; }
+    rc = batch_enqueue(b, b->st);
+    if (rc == HY_OK && (hipEventRecord(g->events[1 + i], b->st) != hipSuccess ||
+                        hipStreamWaitEvent(g->st, g->events[1 + i], 0) != hipSuccess))
+      rc = set_error(HY_ERR_HIP, "hy_group_create: join");
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(g->st, &graph);
+  if (rc != HY_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(rc);
+  }
+  if (ec != hipSuccess) return fail(set_error(HY_ERR_HIP, std::string("hy_group_create: end capture: ") + hipGetErrorString(ec)));
+  g->graph = graph;
+  if (hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+    return fail(set_error(HY_ERR_HIP, "hy_group_create: instantiate"));
+  *out = g;
+  return HY_OK;
+}
+
+int hy_group_launch(hy_group g, int nlaunch) {
+  if (!g || nlaunch < 0) return set_error(HY_ERR_INVALID, "hy_group_launch: bad argument");
+  for (hy_batch b : g->batches) {
+    HY_HIP(hipStreamSynchronize(b->st));
+    for (hy_model h : b->models) HY_HIP(hipStreamSynchronize(h->m.st));
+  }
+  for (int l = 0; l < nlaunch; ++l) HY_HIP(hipGraphLaunch(g->gexec, g->st));
+  HY_HIP(hipEventRecord(g->done, g->st));
+  return HY_OK;
+}
+
+int hy_group_idle(hy_group g, int* idle) {
+  if (!g || !idle) return set_error(HY_ERR_INVALID, "hy_group_idle: null argument");
+  // the event after the last replays, not the stream: a stream that has run
+  // graph launches can read as busy after it drained
+  const hipError_t e = hipEventQuery(g->done);
+  if (e == hipSuccess) { *idle = 1; return HY_OK; }
+  if (e == hipErrorNotReady) { (void)hipGetLastError(); *idle = 0; return HY_OK; }
+  return set_error(HY_ERR_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
+int hy_group_sync(hy_group g) {
+  if (!g) return set_error(HY_ERR_INVALID, "null argument");
+  HY_HIP(hipStreamSynchronize(g->st));
+  return HY_OK;
+}
+
+int hy_group_destroy(hy_group g) {
+  group_free(g);
   return HY_OK;
 }
 

@@ -399,3 +399,38 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Group:
+    """Batches of different shapes (uneven grids) advanced by one graph launch
+    per pass (hy_group_*): each batch is a branch of the graph.  Close the
+    group before its batches."""
+
+    def __init__(self, batches):
+        self.batches = list(batches)
+        hs = (C.c_void_p * len(self.batches))(*[b.h for b in self.batches])
+        h = C.c_void_p()
+        abi.call("hy_group_create", hs, len(self.batches), C.byref(h))
+        self.h = h.value
+
+    def launch(self, nlaunch=1):
+        abi.call("hy_group_launch", self.h, int(nlaunch))
+
+    def idle(self):
+        v = C.c_int()
+        abi.call("hy_group_idle", self.h, C.byref(v))
+        return bool(v.value)
+
+    def sync(self):
+        abi.call("hy_group_sync", self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            abi.call("hy_group_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

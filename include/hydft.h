@@ -270,6 +270,20 @@ int hy_batch_launch(hy_batch b, int nlaunch);
    batch is destroyed. */
 int hy_batch_set_stops(hy_batch b, const int* stops);
 int hy_batch_idle(hy_batch b, int* idle);
+/* Groups of batches: runs on uneven grids advanced by one launch.
+   hy_group_create captures one pass of each batch's launches into one HIP
+   graph, each batch on its own branch (the branches are independent, so the
+   device overlaps them); hy_group_launch enqueues nlaunch replays on the
+   group's stream and returns: every run of every batch then advances by its
+   batch's nsteps per replay (step limits of hy_batch_set_stops hold).  While a
+   group is in flight its batches and models must not be used otherwise;
+   destroy the group before its batches.  No counterpart in the reference. */
+typedef struct hy_group_s* hy_group;
+int hy_group_create(const hy_batch* batches, int n, hy_group* out);
+int hy_group_launch(hy_group g, int nlaunch);
+int hy_group_idle(hy_group g, int* idle);
+int hy_group_sync(hy_group g);
+int hy_group_destroy(hy_group g);
 int hy_batch_sync(hy_batch b);
 int hy_batch_destroy(hy_batch b);
 int hy_model_probe_ring(hy_model m, int ring, double* h_probe);

@@ -181,3 +181,68 @@ def test_batch_runs_of_different_lengths(gpu, precision, closure):
     ref.close()
     for m in ms:
         m.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_group_of_uneven_grids(gpu, precision):
+    """Batches on 16^3 and 24^3 grids (and of both closures) advanced by one
+    graph launch per pass (hy_group_*): every run ends bitwise where it ends
+    alone; step limits set on a member batch hold; the batches work on their
+    own again once the group is closed."""
+    from hydft_amd.model import Group
+    cache = serve._hnc.StructureCache()
+    sets = [_reqs(3, N=16, closure="newtonian", seed=330), _reqs(2, N=24, closure="maxwell", seed=331),
+            _reqs(2, N=24, closure="newtonian", seed=332)]
+    lengths = [[40, 40, 20], [40, 40], [40, 40]]
+    alone = [[None] * len(s) for s in sets]
+    for j, s in enumerate(sets):
+        for i, r in enumerate(s):
+            m = serve.make_model(r, precision, cache)
+            taken, rows = m.advance(lengths[j][i], dt_every=20, probe=r.mode)
+            assert taken == lengths[j][i]
+            alone[j][i] = (m.get_state(), rows, m.time)
+            m.close()
+    models = [[serve.make_model(r, precision, cache) for r in s] for s in sets]
+    batches = [Batch(ms, 20, dt_every=20, probe=(1, 0, 0), ring=40) for ms in models]
+    batches[0].set_stops(lengths[0])
+    g = Group(batches)
+    g.launch(2)
+    g.sync()
+    assert g.idle()
+    for j, ms in enumerate(models):
+        for i, m in enumerate(ms):
+            (na, ma, pa), rows, t = alone[j][i]
+            nb, mb, pb = m.get_state()
+            assert np.array_equal(na, nb) and np.array_equal(ma, mb), (j, i)
+            if pa is not None:
+                assert np.array_equal(pa, pb), (j, i)
+            assert np.array_equal(rows, batches[j].probe(i)[:lengths[j][i]]), (j, i)
+            assert m.time == t, (j, i)
+    g.close()
+    # a member batch launched alone afterwards: 20 more steps equal 60 alone
+    batches[2].launch(1)
+    batches[2].sync()
+    ref = serve.make_model(sets[2][0], precision, cache)
+    ref.advance(60, dt_every=20)
+    assert np.array_equal(models[2][0].get_state()[0], ref.get_state()[0])
+    ref.close()
+    for b in batches:
+        b.close()
+    for ms in models:
+        for m in ms:
+            m.close()
+
+
+@pytest.mark.gpu
+def test_group_rejects_a_batch_twice(gpu):
+    from hydft_amd.model import Group
+    cache = serve._hnc.StructureCache()
+    ms = [serve.make_model(r, "fp64", cache) for r in _reqs(2, closure="newtonian", seed=333)]
+    b = Batch(ms, 20, dt_every=20)
+    with pytest.raises(abi.HydftError) as e:
+        Group([b, b])
+    assert e.value.code == abi.HY_ERR_INVALID
+    b.close()
+    for m in ms:
+        m.close()
