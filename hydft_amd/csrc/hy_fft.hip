@@ -8,6 +8,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include "hy_internal.h"
 #include "hy_regfft.h"
@@ -234,10 +235,19 @@ __device__ inline void gstore_c(void* p, size_t i, cpx<T> c) {
 #endif
 }
 
-template <class T>
-__device__ inline cpx<T> load_any(const void* p, size_t i, int is64) {
-  if (is64) return cvt<T>(gload_c<double>(p, i));
-  return cvt<T>(gload_c<float>(p, i));
+// NP elements idx[i] of the nel-element buffer p (ok[i] false: zero), all
+// loads issued before any is used: one uniform branch on the element type
+// around the group, and branch-free skipped loads (bload_c).
+template <class T, int NP>
+__device__ inline void gather_any(const void* p, size_t nel, const unsigned* idx, const bool* ok, int is64,
+                                  cpx<T>* v) {
+  if (is64) {
+#pragma unroll
+    for (int i = 0; i < NP; ++i) v[i] = cvt<T>(bload_c<double>(p, (unsigned)(nel * sizeof(c64)), idx[i], ok[i]));
+  } else {
+#pragma unroll
+    for (int i = 0; i < NP; ++i) v[i] = cvt<T>(bload_c<float>(p, (unsigned)(nel * sizeof(c32)), idx[i], ok[i]));
+  }
 }
 
 // Block shape of the pencil kernels: NA contiguous columns times TPC threads
@@ -300,21 +310,38 @@ __device__ __attribute__((always_inline)) inline void pencil_inv_body(const Penc
   const PencilField& f = a.f[fi];
   const bool live = ga < a.A;
   const int tailoff = L - a.K;
-  auto load = [&](int p) -> cpx<T> {
-    const int t = kept_index(p, a.h, tailoff);
-    cpx<T> v = mk<T>(0, 0);
-    if (t >= 0 && live) {
-      v = load_any<T>(f.in, ((size_t)b * a.K + t) * a.A + ga, f.in_f64);
-      if (f.k0_offset != 0.0 && t == 0 && ga == 0 && b == 0) v.x = (T)((double)v.x + f.k0_offset);
-      v = apply_mul<T>(v, f.mul, p, ga, a);
+  // the NP elements at positions pos[i] along the pass axis (zero outside the
+  // kept range), with the k = 0 offset and the wavenumber multiplier
+  auto gather = [&](auto np, const int* pos, cpx<T>* v) {
+    constexpr int NP = decltype(np)::value;
+    int t[NP];
+    bool ok[NP];
+    unsigned idx[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      t[i] = kept_index(pos[i], a.h, tailoff);
+      ok[i] = t[i] >= 0 && live;
+      idx[i] = ((unsigned)b * (unsigned)a.K + (unsigned)t[i]) * (unsigned)a.A + (unsigned)ga;
     }
-    return v;
+    gather_any<T, NP>(f.in, (size_t)a.B * a.K * a.A, idx, ok, f.in_f64, v);
+    if (f.k0_offset != 0.0 && ga == 0 && b == 0) {
+#pragma unroll
+      for (int i = 0; i < NP; ++i)
+        if (ok[i] && t[i] == 0) v[i].x = (T)((double)v[i].x + f.k0_offset);
+    }
+    if (f.mul != MUL_NONE) {
+#pragma unroll
+      for (int i = 0; i < NP; ++i)
+        if (ok[i]) v[i] = apply_mul<T>(v[i], f.mul, pos[i], ga, a);
+    }
   };
   void* out = f.out;
   if constexpr (TPC == 1) {
     cpx<T> v[L];
+    int pos[L];
 #pragma unroll
-    for (int p = 0; p < L; ++p) v[p] = load(p);
+    for (int p = 0; p < L; ++p) pos[p] = p;
+    gather(std::integral_constant<int, L>{}, pos, v);
     rdft<T, L, true>(v);
     if (live) {
 #pragma unroll
@@ -326,8 +353,10 @@ __device__ __attribute__((always_inline)) inline void pencil_inv_body(const Penc
     for (int j = 0; j < S::J1; ++j) {
       const int n2 = g + TPC * j;
       cpx<T> v[L1];
+      int pos[L1];
 #pragma unroll
-      for (int n1 = 0; n1 < L1; ++n1) v[n1] = load(L2 * n1 + n2);
+      for (int n1 = 0; n1 < L1; ++n1) pos[n1] = L2 * n1 + n2;
+      gather(std::integral_constant<int, L1>{}, pos, v);
       rdft<T, L1, true>(v);
 #pragma unroll
       for (int k1 = 0; k1 < L1; ++k1) {
@@ -370,8 +399,16 @@ __device__ __attribute__((always_inline)) inline void pencil_fwd_body(const Penc
   const bool live = ga < a.A;
   const int tailoff = L - a.K;
   const T sc = (T)a.scale;
-  auto load = [&](int p) -> cpx<T> {
-    return live ? load_any<T>(f.in, ((size_t)b * L + p) * a.A + ga, f.in_f64) : mk<T>(0, 0);
+  auto gather = [&](auto np, const int* pos, cpx<T>* v) {
+    constexpr int NP = decltype(np)::value;
+    bool ok[NP];
+    unsigned idx[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      ok[i] = live;
+      idx[i] = ((unsigned)b * (unsigned)L + (unsigned)pos[i]) * (unsigned)a.A + (unsigned)ga;
+    }
+    gather_any<T, NP>(f.in, (size_t)a.B * L * a.A, idx, ok, f.in_f64, v);
   };
   void* out = f.out;
   auto store = [&](int p, cpx<T> v) {
@@ -380,8 +417,10 @@ __device__ __attribute__((always_inline)) inline void pencil_fwd_body(const Penc
   };
   if constexpr (TPC == 1) {
     cpx<T> v[L];
+    int pos[L];
 #pragma unroll
-    for (int p = 0; p < L; ++p) v[p] = load(p);
+    for (int p = 0; p < L; ++p) pos[p] = p;
+    gather(std::integral_constant<int, L>{}, pos, v);
     rdft<T, L, false>(v);
 #pragma unroll
     for (int p = 0; p < L; ++p) store(p, v[p]);
@@ -391,8 +430,10 @@ __device__ __attribute__((always_inline)) inline void pencil_fwd_body(const Penc
     for (int j = 0; j < S::J1; ++j) {
       const int n2 = g + TPC * j;
       cpx<T> v[L1];
+      int pos[L1];
 #pragma unroll
-      for (int n1 = 0; n1 < L1; ++n1) v[n1] = load(L2 * n1 + n2);
+      for (int n1 = 0; n1 < L1; ++n1) pos[n1] = L2 * n1 + n2;
+      gather(std::integral_constant<int, L1>{}, pos, v);
       rdft<T, L1, false>(v);
 #pragma unroll
       for (int k1 = 0; k1 < L1; ++k1) {
@@ -717,6 +758,9 @@ int pencil_pass(const Grid& g, int axis, bool inverse, bool trunc, int A, int B,
     a.ay_h = trunc ? (g.Ky + 1) / 2 : g.ny;
     a.ay_L = g.ny;
   }
+  // the kernels address their inputs with 32-bit byte offsets
+  if ((double)A * B * std::max(a.K, a.L) * sizeof(c64) >= 4294967296.0)
+    return set_error(HY_ERR_INVALID, "grid too large: a pencil field must stay below 4 GiB");
   const bool f32 = (g.precision == HY_PREC_MIXED) && trunc;
   for (int i0 = 0; i0 < nf; i0 += MAXPF) {
     int n = std::min(MAXPF, nf - i0);

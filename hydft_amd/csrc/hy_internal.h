@@ -123,6 +123,25 @@ __host__ __device__ inline cpx<T> mul_i(cpx<T> a) { return mk<T>(-a.y, a.x); }  
 template <class TO, class TI>
 __host__ __device__ inline cpx<TO> cvt(cpx<TI> a) { return mk<TO>((TO)a.x, (TO)a.y); }
 
+// Element `elem` of a device buffer of `nbytes` bytes (below 4 GiB) holding
+// cpx<E>, or zero when !ok, without a branch: the offset of a skipped
+// element is the buffer's end, and a raw buffer load past the end returns
+// zero without touching memory.  A group of such loads issues back to back;
+// a branch around each load would make the compiler wait for every load
+// before issuing the next.
+template <class E>
+__device__ inline cpx<E> bload_c(const void* base, unsigned nbytes, unsigned elem, bool ok) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)nbytes, 0x00020000);
+  const unsigned off = ok ? elem * (unsigned)sizeof(cpx<E>) : nbytes;
+  if constexpr (sizeof(E) == 4) return __builtin_bit_cast(cpx<E>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  else return __builtin_bit_cast(cpx<E>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+#else
+  return ok ? reinterpret_cast<const cpx<E>*>(base)[elem] : mk<E>(0, 0);
+#endif
+}
+
 // ------------------------------------------------------ XCD-aware workgroup order
 // Workgroups of a launch are dealt round-robin over the chip's 8 XCDs
 // (workgroup i runs on XCD i % 8), each with its own L2.  Kernels whose

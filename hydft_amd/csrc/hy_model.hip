@@ -225,22 +225,52 @@ struct XIn {
 };
 enum XMul { XM_NONE = 0, XM_IK = 1, XM_NEGK2 = 2 };
 
-template <class T>
-__device__ inline cpx<T> xin_load(const XIn& in, size_t line, int Kx, int kk, double dkx) {
-  const size_t e = line * Kx + kk;
-  cpx<T> v = in.is64 ? cvt<T>(gldc<double>(in.p, e)) : gldc<T>(in.p, e);
-  if (in.k0off != 0.0 && kk == 0 && line == 0) v.x = (T)((double)v.x + in.k0off);
-  // kx of column kk is kk * dkx, computed here: a table load would be a
-  // generic-address load whose wait also drains the wave's pending loads
-  if (in.mul == XM_IK) {
-    const T k = (T)((double)kk * dkx);
-    v = mk<T>(-k * v.y, k * v.x);
-  } else if (in.mul == XM_NEGK2) {
-    const double kd = (double)kk * dkx;
-    const T k2 = (T)(kd * kd);
-    v = mk<T>(-k2 * v.x, -k2 * v.y);
+// The lane's inputs of a first transform stage, gathered before any is used
+// (kx of column kk is kk * dkx, computed: a table load would be a
+// generic-address load whose wait also drains the wave's pending loads):
+// one uniform branch per field on the element type, then every load of the
+// field issued back to back (a wait between them would expose each load's
+// full latency); the kx multipliers and the k = 0 offset follow once all
+// loads are in flight.  ok[i] false: element i is not read (zero).
+template <class T, int NI>
+__device__ inline void xin_gather(const XIn* X, size_t nlines, size_t line, int Kx, const int* kk, const bool* ok,
+                                  cpx<T>* v) {
+#pragma unroll
+  for (int i = 0; i < NI; ++i) v[i] = mk<T>(0, 0);
+  if (!X) return;
+  const unsigned base = (unsigned)(line * Kx);
+  if (X->is64) {
+    const unsigned nb = (unsigned)(nlines * Kx * sizeof(c64));
+#pragma unroll
+    for (int i = 0; i < NI; ++i) v[i] = cvt<T>(bload_c<double>(X->p, nb, base + (unsigned)kk[i], ok[i]));
+  } else {
+    const unsigned nb = (unsigned)(nlines * Kx * sizeof(cpx<T>));
+#pragma unroll
+    for (int i = 0; i < NI; ++i) v[i] = bload_c<T>(X->p, nb, base + (unsigned)kk[i], ok[i]);
   }
-  return v;
+}
+template <class T, int NI>
+__device__ inline void xin_finish(const XIn* X, size_t line, double dkx, const int* kk, const bool* ok, cpx<T>* v) {
+  if (!X) return;
+  if (X->k0off != 0.0 && line == 0) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+      if (ok[i] && kk[i] == 0) v[i].x = (T)((double)v[i].x + X->k0off);
+  }
+  if (X->mul == XM_IK) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const T k = (T)((double)kk[i] * dkx);
+      if (ok[i]) v[i] = mk<T>(-k * v[i].y, k * v[i].x);
+    }
+  } else if (X->mul == XM_NEGK2) {
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+      const double kd = (double)kk[i] * dkx;
+      const T k2 = (T)(kd * kd);
+      if (ok[i]) v[i] = mk<T>(-k2 * v[i].x, -k2 * v[i].y);
+    }
+  }
 }
 
 // ---- x-line transforms on TPC lanes of a wave
@@ -334,24 +364,35 @@ __device__ inline void xfft(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, int lane, 
 // beyond the last (lok false) read nothing.
 template <class T, int NX, int TPC = 64>
 __device__ inline void x_inv_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, const XIn* A, const XIn* B,
-                                  size_t line, bool lok, int Kx, double dkx, int lane, T* qa, T* qb) {
+                                  size_t nlines, size_t line, bool lok, int Kx, double dkx, int lane, T* qa, T* qb) {
 #pragma unroll
   for (int q = 0; q < XQ<NX, TPC>::Q; ++q) {  // slots of idle lanes stay defined
     qa[q] = 0;
     if (qb) qb[q] = 0;
   }
+  // the first stage's operands: element e = j + r NB0 of the line for
+  // j = lane + TPC b, held as i = b R0 + r
+  constexpr int R0 = prad<NX, false>(0), NB0 = NX / R0, NI = ((NB0 + TPC - 1) / TPC) * R0;
+  int kk[NI];
+  bool ok[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int j = lane + TPC * (i / R0), e = j + (i % R0) * NB0;
+    kk[i] = 2 * e > NX ? NX - e : e;
+    ok[i] = lok && (NB0 % TPC == 0 || j < NB0) && kk[i] < Kx;
+  }
+  cpx<T> va[NI], vb[NI];
+  xin_gather<T, NI>(A, nlines, line, Kx, kk, ok, va);
+  xin_gather<T, NI>(B, nlines, line, Kx, kk, ok, vb);
+  xin_finish<T, NI>(A, line, dkx, kk, ok, va);
+  xin_finish<T, NI>(B, line, dkx, kk, ok, vb);
   __syncthreads();  // the previous transform's readers of b0/b1 are done
-  auto in = [&](int, int, int e) -> cpx<T> {
-    const bool hi = 2 * e > NX;
-    const int kk = hi ? NX - e : e;
-    cpx<T> a = mk<T>(0, 0), b = mk<T>(0, 0);
-    if (lok && kk < Kx) {
-      if (A) a = xin_load<T>(*A, line, Kx, kk, dkx);
-      if (B) b = xin_load<T>(*B, line, Kx, kk, dkx);
-    }
-    if (kk == 0 || 2 * kk == NX) { a.y = 0; b.y = 0; }
-    return hi ? mk<T>(a.x + b.y, -a.y + b.x)   // conj(a) + i conj(b)
-              : mk<T>(a.x - b.y, a.y + b.x);   // a + i b
+  auto in = [&](int bb, int r, int e) -> cpx<T> {
+    const int i = bb * R0 + r;
+    cpx<T> a = va[i], b = vb[i];
+    if (kk[i] == 0 || 2 * kk[i] == NX) { a.y = 0; b.y = 0; }
+    return 2 * e > NX ? mk<T>(a.x + b.y, -a.y + b.x)   // conj(a) + i conj(b)
+                      : mk<T>(a.x - b.y, a.y + b.x);   // a + i b
   };
   auto out = [&](int b, int r, int, cpx<T> v) {
     const int q = b * XQ<NX, TPC>::RL + r;
@@ -409,9 +450,9 @@ struct XWave {
     for (int j = lane; j < NX; j += 64) s.tw[swz(j)] = twg[j];
   }
   __device__ static int xpos(int g, int q) { return xown<NX, TPC>(g, q); }
-  __device__ static void inv_pair(Smem& s, const XIn* A, const XIn* B, size_t line, bool lok, int Kx,
-                                  double dkx, int g, int lw, T* qa, T* qb) {
-    x_inv_pair<T, NX, TPC>(s.b0 + lw * NX, s.b1 + lw * NX, s.tw, A, B, line, lok, Kx, dkx, g, qa, qb);
+  __device__ static void inv_pair(Smem& s, const XIn* A, const XIn* B, size_t nlines, size_t line, bool lok,
+                                  int Kx, double dkx, int g, int lw, T* qa, T* qb) {
+    x_inv_pair<T, NX, TPC>(s.b0 + lw * NX, s.b1 + lw * NX, s.tw, A, B, nlines, line, lok, Kx, dkx, g, qa, qb);
   }
   template <class F>
   __device__ static void fwd_pair(Smem& s, F&& val, void* oA, void* oB, size_t line, bool lok, int Kx, int g, int lw) {
@@ -472,7 +513,8 @@ struct PhysArgs {
   const int lane = threadIdx.x;                                                        \
   const int lw = lane / X::TPC, g = lane % X::TPC;                                     \
   const size_t line = (size_t)xcd_work_index(blockIdx.x, gridDim.x) * X::LPW + lw;     \
-  const bool lok = line < (size_t)a.ny * a.nz;                                         \
+  const size_t nlines = (size_t)a.ny * a.nz;                                           \
+  const bool lok = line < nlines;                                                      \
   const int nd = a.nd;                                                                 \
   X::init(sm, twg, lane);
 
@@ -488,20 +530,20 @@ __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { m0[p] = 0; g2[p] = 0; lap[p] = 0; }
   if (a.grad) {
-    X::inv_pair(sm, &a.n, &a.lap, line, lok, a.Kx, a.dkx, g, lw, dn, lap);
+    X::inv_pair(sm, &a.n, &a.lap, nlines, line, lok, a.Kx, a.dkx, g, lw, dn, lap);
     T gx[PTS], gy[PTS];
 #pragma unroll
     for (int p = 0; p < PTS; ++p) gy[p] = 0;
-    X::inv_pair(sm, &a.gx, nd >= 2 ? &a.gy : nullptr, line, lok, a.Kx, a.dkx, g, lw, gx, gy);
+    X::inv_pair(sm, &a.gx, nd >= 2 ? &a.gy : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, gx, gy);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) g2[p] = gx[p] * gx[p] + gy[p] * gy[p];
 #pragma unroll
     for (int p = 0; p < PTS; ++p) gx[p] = 0;
-    X::inv_pair(sm, nd >= 3 ? &a.gz : nullptr, &a.m[0], line, lok, a.Kx, a.dkx, g, lw, gx, m0);
+    X::inv_pair(sm, nd >= 3 ? &a.gz : nullptr, &a.m[0], nlines, line, lok, a.Kx, a.dkx, g, lw, gx, m0);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) g2[p] += gx[p] * gx[p];
   } else {
-    X::inv_pair(sm, &a.n, &a.m[0], line, lok, a.Kx, a.dkx, g, lw, dn, m0);
+    X::inv_pair(sm, &a.n, &a.m[0], nlines, line, lok, a.Kx, a.dkx, g, lw, dn, m0);
   }
   // pointwise functional in FP64
   T mu[PTS], inv[PTS];  // mu_loc, 1 / (m n)
@@ -538,7 +580,7 @@ __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs
   T m1[PTS], m2[PTS];
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { m1[p] = 0; m2[p] = 0; }
-  if (nd >= 2) X::inv_pair(sm, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, line, lok, a.Kx, a.dkx, g, lw, m1, m2);
+  if (nd >= 2) X::inv_pair(sm, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, m1, m2);
   // outputs, one uniform switch per output (not per point)
   auto fill = [&](int c, T* v) {
 #pragma unroll
@@ -578,8 +620,8 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
   T dn[PTS], gmx[PTS], gmy[PTS], gmz[PTS];
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { gmy[p] = 0; gmz[p] = 0; }
-  X::inv_pair(sm, &a.n, &a.mux, line, lok, a.Kx, a.dkx, g, lw, dn, gmx);
-  if (nd >= 2) X::inv_pair(sm, &a.muy, nd >= 3 ? &a.muz : nullptr, line, lok, a.Kx, a.dkx, g, lw, gmy, gmz);
+  X::inv_pair(sm, &a.n, &a.mux, nlines, line, lok, a.Kx, a.dkx, g, lw, dn, gmx);
+  if (nd >= 2) X::inv_pair(sm, &a.muy, nd >= 3 ? &a.muz : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, gmy, gmz);
   // external force (external_potential.f90:65-93) at the stage-start time,
   // evaluated point by point
   const bool wave_f = a.vext_kind == HY_VEXT_MODE || a.vext_kind == HY_VEXT_DRIVEN;
@@ -630,8 +672,8 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
     T m0[PTS], m1[PTS], m2[PTS];
 #pragma unroll
     for (int p = 0; p < PTS; ++p) { m1[p] = 0; m2[p] = 0; }
-    X::inv_pair(sm, &a.m[0], nd >= 2 ? &a.m[1] : nullptr, line, lok, a.Kx, a.dkx, g, lw, m0, m1);
-    if (nd >= 3) X::inv_pair(sm, &a.m[2], nullptr, line, lok, a.Kx, a.dkx, g, lw, m2, nullptr);
+    X::inv_pair(sm, &a.m[0], nd >= 2 ? &a.m[1] : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, m0, m1);
+    if (nd >= 3) X::inv_pair(sm, &a.m[2], nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, m2, nullptr);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) {
       const double inv = fd::rcp_nr(a.mass * (a.n0 + dn[p]));
@@ -643,11 +685,11 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
 #pragma unroll
     for (int p = 0; p < PTS; ++p) { ya[p] = 0; za[p] = 0; yb[p] = 0; zb[p] = 0; }
     if (nd == 1) {
-      X::inv_pair(sm, &a.px[c], &a.px[c + 1], line, lok, a.Kx, a.dkx, g, lw, xa, xb);
+      X::inv_pair(sm, &a.px[c], &a.px[c + 1], nlines, line, lok, a.Kx, a.dkx, g, lw, xa, xb);
     } else {
-      X::inv_pair(sm, &a.px[c], &a.py[c], line, lok, a.Kx, a.dkx, g, lw, xa, ya);
-      X::inv_pair(sm, &a.px[c + 1], &a.py[c + 1], line, lok, a.Kx, a.dkx, g, lw, xb, yb);
-      if (nd >= 3) X::inv_pair(sm, &a.pz[c], &a.pz[c + 1], line, lok, a.Kx, a.dkx, g, lw, za, zb);
+      X::inv_pair(sm, &a.px[c], &a.py[c], nlines, line, lok, a.Kx, a.dkx, g, lw, xa, ya);
+      X::inv_pair(sm, &a.px[c + 1], &a.py[c + 1], nlines, line, lok, a.Kx, a.dkx, g, lw, xb, yb);
+      if (nd >= 3) X::inv_pair(sm, &a.pz[c], &a.pz[c + 1], nlines, line, lok, a.Kx, a.dkx, g, lw, za, zb);
     }
     X::fwd_pair(
         sm,
@@ -668,8 +710,8 @@ __device__ __attribute__((always_inline)) inline void umax_body(const PhysArgs* 
   T dn[PTS], m0[PTS], m1[PTS], m2[PTS];
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { m1[p] = 0; m2[p] = 0; }
-  X::inv_pair(sm, &a.n, &a.m[0], line, lok, a.Kx, a.dkx, g, lw, dn, m0);
-  if (nd >= 2) X::inv_pair(sm, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, line, lok, a.Kx, a.dkx, g, lw, m1, m2);
+  X::inv_pair(sm, &a.n, &a.m[0], nlines, line, lok, a.Kx, a.dkx, g, lw, dn, m0);
+  if (nd >= 2) X::inv_pair(sm, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, m1, m2);
   double um = 0.0;
 #pragma unroll
   for (int p = 0; p < PTS; ++p) {
@@ -1377,6 +1419,9 @@ static int check_xk_inputs(const Model& m, int which, const PhysArgs& a) {
   if (which != XK_UMAX)
     for (int o = 0; o < a.nout; ++o) ok = ok && a.dst[o] != nullptr;
   if (!ok) return set_error(HY_ERR_INVALID, "internal: an x-line kernel input or output is not set");
+  // the kernels address their inputs with 32-bit byte offsets
+  if ((double)a.Kx * a.ny * a.nz * sizeof(c64) >= 4294967296.0)
+    return set_error(HY_ERR_INVALID, "grid too large: an x-line field must stay below 4 GiB");
   return HY_OK;
 }
 
