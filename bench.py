@@ -115,11 +115,12 @@ def copy_probe(mib):
     return 0
 
 
-# kernel-class name -> device kernel of that class (pencil classes share one
-# kernel per direction across the y and z axes)
-CLASS_KERNEL = {"x_phaseA": "k_phaseA", "x_phaseB": "k_phaseB", "spec_update": "k_spec2", "spec_mu": "k_spec1",
-                "pencil_z_inverse": "k_pencil_inv", "pencil_y_inverse": "k_pencil_inv",
-                "pencil_y_forward": "k_pencil_fwd", "pencil_z_forward": "k_pencil_fwd", "x_umax": "k_umax"}
+# kernel-class name -> device kernel of that class, and the tag in its name
+# (the single-run pencil kernels carry their axis: AxisY or AxisZ)
+CLASS_KERNEL = {"x_phaseA": ("k_phaseA", ""), "x_phaseB": ("k_phaseB", ""), "spec_update": ("k_spec2", ""),
+                "spec_mu": ("k_spec1", ""), "pencil_z_inverse": ("k_pencil_inv", "AxisZ"),
+                "pencil_y_inverse": ("k_pencil_inv", "AxisY"), "pencil_y_forward": ("k_pencil_fwd", "AxisY"),
+                "pencil_z_forward": ("k_pencil_fwd", "AxisZ"), "x_umax": ("k_umax", "")}
 COPY_MIB = 1024
 
 
@@ -181,11 +182,12 @@ def pmc_traffic(args):
 
 def kernel_traffic(tr, cls, precision):
     """(fetch + write) bytes per launch of the device kernel of class `cls` in
-    the run's transform type (pencil classes: mean over the y and z passes)."""
+    the run's transform type (mean over the dispatches of that kernel)."""
     if not tr or cls not in CLASS_KERNEL:
         return None
     want = "<float" if precision == "mixed" else "<double"
-    names = [k for k in tr["per_kernel"] if CLASS_KERNEL[cls] + "<" in k]
+    kern, tag = CLASS_KERNEL[cls]
+    names = [k for k in tr["per_kernel"] if kern + "<" in k and tag in k]
     names = [k for k in names if want in k] or names
     for name in names:
         c = tr["per_kernel"][name]

@@ -456,7 +456,13 @@ __device__ __attribute__((always_inline)) inline void pencil_fwd_body(const Penc
   }
 }
 
-template <class T, int L, int NA>
+// Axis tags of the single-run pencil kernels: the y and z passes run kernels
+// of their own names, so kernel traces and counters tell the two axes apart
+// (the batched kernels serve both).
+struct AxisY {};
+struct AxisZ {};
+
+template <class T, int L, int NA, class AX>
 __global__ __launch_bounds__((PencilShape<T, L, NA>::NT)) void k_pencil_inv(PencilK<T> k) {
   pencil_inv_body<T, L, NA>(k.a, k.tw);
 }
@@ -464,7 +470,7 @@ template <class T, int L, int NA>
 __global__ __launch_bounds__((PencilShape<T, L, NA>::NT)) void k_pencil_inv_b(const PencilK<T>* __restrict__ ks) {
   pencil_inv_body<T, L, NA>(ks[blockIdx.z].a, ks[blockIdx.z].tw);
 }
-template <class T, int L, int NA>
+template <class T, int L, int NA, class AX>
 __global__ __launch_bounds__((PencilShape<T, L, NA>::NT)) void k_pencil_fwd(PencilK<T> k) {
   pencil_fwd_body<T, L, NA>(k.a, k.tw);
 }
@@ -480,11 +486,15 @@ struct PencilPlanFns {
   unsigned nt = 0;
 };
 template <class T, int L, int NA>
-static PencilPlanFns pencil_fns_na(bool inverse) {
+static PencilPlanFns pencil_fns_na(bool inverse, bool y) {
   PencilPlanFns p;
   if constexpr (pencil_na_ok<T, L, NA>()) {
-    p.single = inverse ? reinterpret_cast<const void*>(&k_pencil_inv<T, L, NA>)
-                       : reinterpret_cast<const void*>(&k_pencil_fwd<T, L, NA>);
+    if (inverse)
+      p.single = y ? reinterpret_cast<const void*>(&k_pencil_inv<T, L, NA, AxisY>)
+                   : reinterpret_cast<const void*>(&k_pencil_inv<T, L, NA, AxisZ>);
+    else
+      p.single = y ? reinterpret_cast<const void*>(&k_pencil_fwd<T, L, NA, AxisY>)
+                   : reinterpret_cast<const void*>(&k_pencil_fwd<T, L, NA, AxisZ>);
     p.batched = inverse ? reinterpret_cast<const void*>(&k_pencil_inv_b<T, L, NA>)
                         : reinterpret_cast<const void*>(&k_pencil_fwd_b<T, L, NA>);
     p.nt = (unsigned)PencilShape<T, L, NA>::NT;
@@ -492,21 +502,22 @@ static PencilPlanFns pencil_fns_na(bool inverse) {
   return p;
 }
 template <class T, int L>
-static PencilPlanFns pencil_fns_l(bool inverse, int na) {
+static PencilPlanFns pencil_fns_l(bool inverse, int na, bool y) {
   switch (na) {
-    case 8: return pencil_fns_na<T, L, 8>(inverse);
-    case 16: return pencil_fns_na<T, L, 16>(inverse);
-    case 24: return pencil_fns_na<T, L, 24>(inverse);
-    case 32: return pencil_fns_na<T, L, 32>(inverse);
-    case 64: return pencil_fns_na<T, L, 64>(inverse);
-    case 128: return pencil_fns_na<T, L, 128>(inverse);
+    case 8: return pencil_fns_na<T, L, 8>(inverse, y);
+    case 16: return pencil_fns_na<T, L, 16>(inverse, y);
+    case 24: return pencil_fns_na<T, L, 24>(inverse, y);
+    case 32: return pencil_fns_na<T, L, 32>(inverse, y);
+    case 64: return pencil_fns_na<T, L, 64>(inverse, y);
+    case 128: return pencil_fns_na<T, L, 128>(inverse, y);
   }
   return PencilPlanFns{};
 }
+// y: the single-run kernel of the y pass (else the z pass)
 template <class T>
-static PencilPlanFns pencil_fns(int L, bool inverse, int na) {
+static PencilPlanFns pencil_fns(int L, bool inverse, int na, bool y) {
   switch (L) {
-#define HY_PL(LL) case LL: return pencil_fns_l<T, LL>(inverse, na);
+#define HY_PL(LL) case LL: return pencil_fns_l<T, LL>(inverse, na, y);
     HY_PL(4) HY_PL(8) HY_PL(12) HY_PL(16) HY_PL(24) HY_PL(32) HY_PL(48) HY_PL(64) HY_PL(96)
     HY_PL(128) HY_PL(256)
 #undef HY_PL
@@ -653,7 +664,7 @@ static int choose_na(const PencilArgs& a, bool inverse, int nf, const cpx<T>* tw
   std::vector<Cand> cands;
   std::vector<int> nas;
   for (int na : kNaCandidates) {
-    const PencilPlanFns p = pencil_fns<T>(a.L, inverse, na);
+    const PencilPlanFns p = pencil_fns<T>(a.L, inverse, na, a.y_from_pencil != 0);
     if (!p.single) continue;
     cands.push_back({p.single, dim3(pencil_grid_x<T>(a, nf, na)), dim3(p.nt)});
     nas.push_back(na);
@@ -686,7 +697,7 @@ static int batch_plan_t(const std::array<int, 8>& pk, int nz, void* d_args, hipS
       std::vector<Cand> cands;
       std::vector<int> nas;
       for (int c : kNaCandidates) {
-        const PencilPlanFns p = pencil_fns<T>(L, inv != 0, c);
+        const PencilPlanFns p = pencil_fns<T>(L, inv != 0, c, false);
         if (!p.batched) continue;
         cands.push_back({p.batched, dim3(pencil_grid_x<T>(a, nf, c), 1, (unsigned)nz), dim3(p.nt)});
         nas.push_back(c);
@@ -697,7 +708,7 @@ static int batch_plan_t(const std::array<int, 8>& pk, int nz, void* d_args, hipS
       plan_store(key, na);
     }
   }
-  const PencilPlanFns p = pencil_fns<T>(L, inv != 0, na);
+  const PencilPlanFns p = pencil_fns<T>(L, inv != 0, na, false);
   if (!p.batched) return set_error(HY_ERR_INVALID, "pencil plan: no kernel for the chosen shape");
   *fn = p.batched;
   *grid = dim3(pencil_grid_x<T>(a, nf, na), 1, (unsigned)nz);
@@ -713,7 +724,7 @@ int pencil_batch_plan(const std::array<int, 8>& pk, int nz, void* d_args, hipStr
 template <class T>
 static int dispatch_pencil(const PencilArgs& a, bool inverse, int nf, const cpx<T>* tw, hipStream_t s) {
   const int na = choose_na<T>(a, inverse, nf, tw, s);
-  const PencilPlanFns p = pencil_fns<T>(a.L, inverse, na);
+  const PencilPlanFns p = pencil_fns<T>(a.L, inverse, na, a.y_from_pencil != 0);
   if (!p.single) return set_error(HY_ERR_UNSUPPORTED, "pencil: unsupported length");
   const PencilK<T> k{a, tw};
   if (Recorder* r = recorder()) {

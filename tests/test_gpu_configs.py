@@ -29,6 +29,9 @@ pytestmark = pytest.mark.gpu
 
 TOL64 = 1e-11
 MIXED_TOL_N, MIXED_TOL_DN = 1e-6, 1e-4
+# the Maxwell stress tensor as a whole in mixed precision: it is formed from
+# FP32 velocity gradients (one more factor k than mom)
+MIXED_TOL_PI = 1e-3
 TWOPI = 2.0 * math.pi
 
 
@@ -164,6 +167,9 @@ def _errors(p, sp, got, ref):
         for c in range(6):
             if np.linalg.norm(rpi[c]) > 0:
                 e[f"pi{c}"] = P.rel_l2(pi[c], rpi[c])
+        # the tensor as a whole: FP32 round-off is relative to its largest
+        # components, so its small components carry large relative errors
+        e["pi"] = float(np.linalg.norm(np.asarray(pi) - np.asarray(rpi)) / np.linalg.norm(np.asarray(rpi)))
     return e
 
 
@@ -187,6 +193,8 @@ def test_config_shapes_mixed(gpu, name):
     for k, v in e.items():
         if k == "dn" or k.startswith("mom"):
             assert v <= MIXED_TOL_DN, (name, k, v)
+    if "pi" in e:
+        assert e["pi"] <= MIXED_TOL_PI, (name, e["pi"])
 
 
 # ---------------------------------------------------------------- config 1
