@@ -5,8 +5,9 @@
 !> and Hartree terms and no viscosity, so that linear theory is the closed
 !> form omega = q sqrt(n0 K(q) / m), K(q) = 1/(beta n0) + 4 pi e2/(q^2 + kappa^2).
 !> Checks: frequency within 1 % of linear theory, mass conserved to 1e-12
-!> (test_conservation.f90), the kept 2/3-rule box, and batched and captured
-!> steps equal to hy_model_advance bitwise.
+!> (test_conservation.f90), the kept 2/3-rule box, batched and captured
+!> steps equal to hy_model_advance bitwise, and a group of a batch whose runs
+!> have different step limits.
 program drv_model
   use, intrinsic :: iso_c_binding
   use hydft_hip_c
@@ -17,9 +18,9 @@ program drv_model
   real(c_double), parameter :: gamma = 2.0_c_double, kappa = 0.1_c_double, lx = 20.0_c_double
   type(hy_term_desc), target :: terms(2)
   type(hy_model_desc) :: desc
-  type(c_ptr) :: m, ms(2), ma, mg, b
-  type(c_ptr), target :: pair(2)
-  integer(c_int), target :: probe3(3)
+  type(c_ptr) :: m, ms(2), ma, mg, b, g
+  type(c_ptr), target :: pair(2), grp(1)
+  integer(c_int), target :: probe3(3), stops(2)
   real(c_double), allocatable, target :: rows(:,:), rows_a(:,:), rows_b(:,:), rows_g(:,:)
   real(c_double), target :: n(nx), mom(nx, 3)
   real(c_double) :: n0, beta, e2, q, kq, om_lin, om, t, dt, bud0(6), bud1(6), err
@@ -102,6 +103,34 @@ program drv_model
   call hy_check(hy_model_graph_probe(mg, c_loc(rows_g)), 'graph_probe')
   call hy_check(hy_model_probe_ring(ms(2), 40_c_int, c_loc(rows_b)), 'probe_ring')
   call check(all(rows_b == rows_g), 'batch = graph', maxval(abs(rows_b - rows_g)))
+  call hy_check(hy_batch_destroy(b), 'batch_destroy')
+  do i = 1, 2
+    call hy_check(hy_model_destroy(ms(i)), 'destroy')
+  end do
+
+  ! ---- a group launch of a batch whose runs stop at 20 and 40 steps
+  call hy_check(hy_model_create(desc, ms(1)), 'model_create')
+  call hy_check(hy_model_create(desc, ms(2)), 'model_create')
+  call set_mode(ms(1), 1.0e-4_c_double)
+  call set_mode(ms(2), 1.0e-4_c_double)
+  pair = ms
+  stops = [20_c_int, 40_c_int]
+  call hy_check(hy_batch_create(c_loc(pair), 2_c_int, 20_c_int, 20_c_int, c_loc(probe3), 40_c_int, b), 'batch_create')
+  call hy_check(hy_batch_set_stops(b, c_loc(stops)), 'batch_set_stops')
+  grp(1) = b
+  call hy_check(hy_group_create(c_loc(grp), 1_c_int, g), 'group_create')
+  call hy_check(hy_group_launch(g, 2_c_int), 'group_launch')
+  call hy_check(hy_group_sync(g), 'group_sync')
+  call hy_check(hy_group_idle(g, idle), 'group_idle')
+  call check(idle == 1, 'group idle after sync', real(idle, c_double))
+  call hy_check(hy_group_destroy(g), 'group_destroy')
+  call hy_check(hy_model_probe_ring(ms(2), 40_c_int, c_loc(rows_b)), 'probe_ring')
+  call check(all(rows_b == rows_a), 'group = advance', maxval(abs(rows_b - rows_a)))
+  call hy_check(hy_model_probe_ring(ms(1), 40_c_int, c_loc(rows_b)), 'probe_ring')
+  call check(all(rows_b(:, 1:20) == rows_a(:, 1:20)), 'group, 20-step run', &
+             maxval(abs(rows_b(:, 1:20) - rows_a(:, 1:20))))
+  call hy_check(hy_model_get_time(ms(1), t, dt, step), 'get_time')
+  call check(step == 20, 'run stopped at its limit', real(step, c_double))
   call hy_check(hy_batch_destroy(b), 'batch_destroy')
   do i = 1, 2
     call hy_check(hy_model_destroy(ms(i)), 'destroy')

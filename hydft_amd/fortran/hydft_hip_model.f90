@@ -153,6 +153,37 @@ module hydft_hip_model
       import :: c_int, c_ptr
       type(c_ptr), value :: b
     end function
+    !> per-run step limits of a batch (stops(i): steps of run i, a multiple of the
+    !> batch's steps per launch; 0: none)
+    integer(c_int) function hy_batch_set_stops(b, stops) bind(C, name='hy_batch_set_stops')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b, stops
+    end function
+    ! groups of batches on different grids: one graph launch per pass
+    integer(c_int) function hy_group_create(batches, n, g) bind(C, name='hy_group_create')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batches
+      integer(c_int), value :: n
+      type(c_ptr), intent(out) :: g
+    end function
+    integer(c_int) function hy_group_launch(g, nlaunch) bind(C, name='hy_group_launch')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: g
+      integer(c_int), value :: nlaunch
+    end function
+    integer(c_int) function hy_group_idle(g, idle) bind(C, name='hy_group_idle')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: g
+      integer(c_int), intent(out) :: idle
+    end function
+    integer(c_int) function hy_group_sync(g) bind(C, name='hy_group_sync')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: g
+    end function
+    integer(c_int) function hy_group_destroy(g) bind(C, name='hy_group_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: g
+    end function
     integer(c_int) function hy_model_probe_ring(m, ring, probe) bind(C, name='hy_model_probe_ring')
       import :: c_int, c_ptr
       type(c_ptr), value :: m, probe

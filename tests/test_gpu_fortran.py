@@ -27,11 +27,12 @@ EXE_MODEL = os.path.join(os.path.dirname(EXE), "f_model")
 def test_fortran_model_drop_in(gpu):
     """The model entry points through hydft_hip_model: a single-mode ion run
     against linear theory (1 %), mass conservation (1e-12), the kept 2/3-rule
-    box, and batched and captured steps equal to hy_model_advance bitwise."""
+    box, batched and captured steps equal to hy_model_advance bitwise, and a
+    group launch of a batch whose runs stop at 20 and 40 steps."""
     if not os.path.exists(EXE_MODEL):
         pytest.fail("Fortran program not built (run __graft_entry__.build())")
     r = subprocess.run([EXE_MODEL], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "drv_model: all passed" in r.stdout
-    assert r.stdout.count("  pass  ") == 5
+    assert r.stdout.count("  pass  ") == 9
