@@ -232,14 +232,22 @@ enum XMul { XM_NONE = 0, XM_IK = 1, XM_NEGK2 = 2 };
 // field issued back to back (a wait between them would expose each load's
 // full latency); the kx multipliers and the k = 0 offset follow once all
 // loads are in flight.  ok[i] false: element i is not read (zero).
-template <class T, int NI>
+//
+// M >= 0: the input is a compute-type x-line slot with no k = 0 offset and
+// the multiplier M (an XMul), known at compile time (every input of a 3-D
+// grid); M = -1: element type, offset and multiplier are read from X.
+template <class T, int NI, int M = -1>
 __device__ inline void xin_gather(const XIn* X, size_t nlines, size_t line, int Kx, const int* kk, const bool* ok,
                                   cpx<T>* v) {
 #pragma unroll
   for (int i = 0; i < NI; ++i) v[i] = mk<T>(0, 0);
   if (!X) return;
   const unsigned base = (unsigned)(line * Kx);
-  if (X->is64) {
+  if constexpr (M >= 0) {
+    const unsigned nb = (unsigned)(nlines * Kx * sizeof(cpx<T>));
+#pragma unroll
+    for (int i = 0; i < NI; ++i) v[i] = bload_c<T>(X->p, nb, base + (unsigned)kk[i], ok[i]);
+  } else if (X->is64) {
     const unsigned nb = (unsigned)(nlines * Kx * sizeof(c64));
 #pragma unroll
     for (int i = 0; i < NI; ++i) v[i] = cvt<T>(bload_c<double>(X->p, nb, base + (unsigned)kk[i], ok[i]));
@@ -249,9 +257,26 @@ __device__ inline void xin_gather(const XIn* X, size_t nlines, size_t line, int 
     for (int i = 0; i < NI; ++i) v[i] = bload_c<T>(X->p, nb, base + (unsigned)kk[i], ok[i]);
   }
 }
-template <class T, int NI>
+template <class T, int NI, int M = -1>
 __device__ inline void xin_finish(const XIn* X, size_t line, double dkx, const int* kk, const bool* ok, cpx<T>* v) {
   if (!X) return;
+  if constexpr (M >= 0) {
+    if constexpr (M == XM_IK) {
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        const T k = (T)((double)kk[i] * dkx);
+        v[i] = mk<T>(-k * v[i].y, k * v[i].x);  // skipped elements are zero
+      }
+    } else if constexpr (M == XM_NEGK2) {
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        const double kd = (double)kk[i] * dkx;
+        const T k2 = (T)(kd * kd);
+        v[i] = mk<T>(-k2 * v[i].x, -k2 * v[i].y);
+      }
+    }
+    return;
+  }
   if (X->k0off != 0.0 && line == 0) {
 #pragma unroll
     for (int i = 0; i < NI; ++i)
@@ -361,8 +386,9 @@ __device__ inline void xfft(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, int lane, 
 // Inverse x transform of two spectral x-lines packed as a + i b (a c2r pair:
 // DC and Nyquist imaginary parts dropped); the lane's points of a(x) and
 // b(x) are returned in qa[q], qb[q] (qb may be null).  Lanes of a line
-// beyond the last (lok false) read nothing.
-template <class T, int NX, int TPC = 64>
+// beyond the last (lok false) read nothing.  MA, MB: the inputs' compile-time
+// multipliers (see xin_gather), or -1.
+template <class T, int NX, int TPC = 64, int MA = -1, int MB = -1>
 __device__ inline void x_inv_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, const XIn* A, const XIn* B,
                                   size_t nlines, size_t line, bool lok, int Kx, double dkx, int lane, T* qa, T* qb) {
 #pragma unroll
@@ -382,10 +408,10 @@ __device__ inline void x_inv_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, cons
     ok[i] = lok && (NB0 % TPC == 0 || j < NB0) && kk[i] < Kx;
   }
   cpx<T> va[NI], vb[NI];
-  xin_gather<T, NI>(A, nlines, line, Kx, kk, ok, va);
-  xin_gather<T, NI>(B, nlines, line, Kx, kk, ok, vb);
-  xin_finish<T, NI>(A, line, dkx, kk, ok, va);
-  xin_finish<T, NI>(B, line, dkx, kk, ok, vb);
+  xin_gather<T, NI, MA>(A, nlines, line, Kx, kk, ok, va);
+  xin_gather<T, NI, MB>(B, nlines, line, Kx, kk, ok, vb);
+  xin_finish<T, NI, MA>(A, line, dkx, kk, ok, va);
+  xin_finish<T, NI, MB>(B, line, dkx, kk, ok, vb);
   __syncthreads();  // the previous transform's readers of b0/b1 are done
   auto in = [&](int bb, int r, int e) -> cpx<T> {
     const int i = bb * R0 + r;
@@ -450,9 +476,10 @@ struct XWave {
     for (int j = lane; j < NX; j += 64) s.tw[swz(j)] = twg[j];
   }
   __device__ static int xpos(int g, int q) { return xown<NX, TPC>(g, q); }
+  template <int MA = -1, int MB = -1>
   __device__ static void inv_pair(Smem& s, const XIn* A, const XIn* B, size_t nlines, size_t line, bool lok,
                                   int Kx, double dkx, int g, int lw, T* qa, T* qb) {
-    x_inv_pair<T, NX, TPC>(s.b0 + lw * NX, s.b1 + lw * NX, s.tw, A, B, nlines, line, lok, Kx, dkx, g, qa, qb);
+    x_inv_pair<T, NX, TPC, MA, MB>(s.b0 + lw * NX, s.b1 + lw * NX, s.tw, A, B, nlines, line, lok, Kx, dkx, g, qa, qb);
   }
   template <class F>
   __device__ static void fwd_pair(Smem& s, F&& val, void* oA, void* oB, size_t line, bool lok, int Kx, int g, int lw) {
@@ -505,8 +532,27 @@ struct PhysArgs {
   void* dst[10];
 };
 
+// Compile-time configuration of the x-line kernels.  XCfg<XF_RUNTIME> reads
+// every choice from the argument block.  The other forms are for 3-D grids,
+// whose x-line inputs are all compute-type slots with no k = 0 offset and a
+// multiplier fixed by their role (XMul): XF_ANY3 fixes that layout (phase B
+// of any model); XF_IDEAL also fixes phase A of a classical ion model (the
+// ideal term as the only pointwise functional; mu, u and the six fluxes
+// emitted in the order the host lists them).  Fixing these at compile time
+// removes the uniform branches on the inputs' types and on the output codes,
+// with the register moves at their joins.  (A fixed form of the electron
+// functional was tried: its straight-line code needed more than the 128
+// VGPRs of 4 waves per SIMD and spilled.)
+enum XFun { XF_RUNTIME = 0, XF_IDEAL = 1, XF_ANY3 = 2 };
+template <int F>
+struct XCfg {
+  static constexpr bool D3 = F != XF_RUNTIME;
+  static constexpr bool IDEAL = F == XF_IDEAL;
+  static constexpr int MN = D3 ? XM_NONE : -1, MK = D3 ? XM_IK : -1;
+};
+
 // Common preamble of the x-line kernels: the lane's line and position in it.
-#define HY_XLINE_PREAMBLE                                                              \
+#define HY_XLINE_PREAMBLE(ND3)                                                         \
   constexpr int PTS = X::PTS;                                                          \
   __shared__ typename X::Smem sm;                                                      \
   const PhysArgs& a = *pa;                                                             \
@@ -515,12 +561,17 @@ struct PhysArgs {
   const size_t line = (size_t)xcd_work_index(blockIdx.x, gridDim.x) * X::LPW + lw;     \
   const size_t nlines = (size_t)a.ny * a.nz;                                           \
   const bool lok = line < nlines;                                                      \
-  const int nd = a.nd;                                                                 \
+  const int nd = (ND3) ? 3 : a.nd;                                                     \
   X::init(sm, twg, lane);
 
-template <class T, int NX, class X>
+template <class T, int NX, class X, class C>
 __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
-  HY_XLINE_PREAMBLE
+  constexpr bool FIX = C::IDEAL;
+  HY_XLINE_PREAMBLE(FIX)
+  constexpr int MN = FIX ? XM_NONE : -1, MK = FIX ? XM_IK : -1;
+  const bool ideal = FIX || a.ideal != 0;
+  const bool tf = !FIX && a.tf != 0;
+  const bool grad = !FIX && a.grad != 0;
   // per-point values are held in the transform type T: in mixed precision
   // they come out of (and go back into) FP32 transforms; the pointwise
   // functional below is evaluated in FP64
@@ -529,21 +580,23 @@ __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs
   T dn[PTS], m0[PTS], g2[PTS], lap[PTS];
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { m0[p] = 0; g2[p] = 0; lap[p] = 0; }
-  if (a.grad) {
-    X::inv_pair(sm, &a.n, &a.lap, nlines, line, lok, a.Kx, a.dkx, g, lw, dn, lap);
+  if (grad) {
+    X::template inv_pair<MN, MN>(sm, &a.n, &a.lap, nlines, line, lok, a.Kx, a.dkx, g, lw, dn, lap);
     T gx[PTS], gy[PTS];
 #pragma unroll
     for (int p = 0; p < PTS; ++p) gy[p] = 0;
-    X::inv_pair(sm, &a.gx, nd >= 2 ? &a.gy : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, gx, gy);
+    X::template inv_pair<MK, MN>(sm, &a.gx, nd >= 2 ? &a.gy : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, gx,
+                                 gy);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) g2[p] = gx[p] * gx[p] + gy[p] * gy[p];
 #pragma unroll
     for (int p = 0; p < PTS; ++p) gx[p] = 0;
-    X::inv_pair(sm, nd >= 3 ? &a.gz : nullptr, &a.m[0], nlines, line, lok, a.Kx, a.dkx, g, lw, gx, m0);
+    X::template inv_pair<MN, MN>(sm, nd >= 3 ? &a.gz : nullptr, &a.m[0], nlines, line, lok, a.Kx, a.dkx, g, lw,
+                                 gx, m0);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) g2[p] += gx[p] * gx[p];
   } else {
-    X::inv_pair(sm, &a.n, &a.m[0], nlines, line, lok, a.Kx, a.dkx, g, lw, dn, m0);
+    X::template inv_pair<MN, MN>(sm, &a.n, &a.m[0], nlines, line, lok, a.Kx, a.dkx, g, lw, dn, m0);
   }
   // pointwise functional in FP64
   T mu[PTS], inv[PTS];  // mu_loc, 1 / (m n)
@@ -557,14 +610,14 @@ __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs
     inv[p] = (T)fd::rcp_nr(a.mass * n);
     // mu_loc with its k = 0 constants (ln n0 / beta, alpha0 / beta) dropped:
     // only grad(mu) enters the momentum equation (model.f90:99).
-    double v = a.ideal ? log1p((double)dn[p] * a.inv_n0) * a.inv_beta : 0.0;  // fe_ideal_classical.f90:40
-    if (a.tf || a.grad) {
+    double v = ideal ? log1p((double)dn[p] * a.inv_n0) * a.inv_beta : 0.0;  // fe_ideal_classical.f90:40
+    if (tf || grad) {
       const double al = live && n > 0.0
                             ? fd::alpha_from_ip12_dev(a.fdtab, n * a.inv_cn,
                                                       a.alpha0 + (double)dn[p] * (a.dadn + 0.5 * a.d2adn2 * (double)dn[p]))
                             : a.alpha0;
-      if (a.tf) v += (al - a.alpha0) * a.inv_beta;  // fe_kinetic_tfk.f90:80-81
-      if (a.grad) {                                 // fe_kinetic_tfk.f90:130-137
+      if (tf) v += (al - a.alpha0) * a.inv_beta;  // fe_kinetic_tfk.f90:80-81
+      if (grad) {                                 // fe_kinetic_tfk.f90:130-137
         double im, d1, d2;
         fd::im12_d1_d2_dev(a.fdtab, al, im, d1, d2);
         const double rim = fd::rcp_nr(im);
@@ -575,12 +628,14 @@ __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs
       }
     }
     mu[p] = (T)v;
-  }
+    }
   if (bad) a.ctl->nonpos = 1;
   T m1[PTS], m2[PTS];
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { m1[p] = 0; m2[p] = 0; }
-  if (nd >= 2) X::inv_pair(sm, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, m1, m2);
+  if (nd >= 2)
+    X::template inv_pair<MN, MN>(sm, &a.m[1], nd >= 3 ? &a.m[2] : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw,
+                                 m1, m2);
   // outputs, one uniform switch per output (not per point)
   auto fill = [&](int c, T* v) {
 #pragma unroll
@@ -603,25 +658,43 @@ __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs
       default: break;
     }
   };
-  for (int o = 0; o < a.nout; o += 2) {
-    const bool two = o + 1 < a.nout;
-    T va[PTS], vb[PTS];
-    fill(a.code[o], va);
-    fill(two ? a.code[o + 1] : -1, vb);
-    X::fwd_pair(sm, [&](int q) { return mk<T>(va[q], vb[q]); }, a.dst[o], two ? a.dst[o + 1] : nullptr, line,
-                lok, a.Kx, g, lw);
+  if constexpr (FIX) {
+    // the host lists mu, u1, u2, u3, F11, F12, F13, F22, F23, F33
+    auto pair = [&](int o, int ca, int cb) {
+      T va[PTS], vb[PTS];
+      fill(ca, va);
+      fill(cb, vb);
+      X::fwd_pair(sm, [&](int q) { return mk<T>(va[q], vb[q]); }, a.dst[o], a.dst[o + 1], line, lok, a.Kx, g, lw);
+    };
+    pair(0, OA_MU, OA_U1);
+    pair(2, OA_U2, OA_U3);
+    pair(4, OA_F11, OA_F12);
+    pair(6, OA_F13, OA_F22);
+    pair(8, OA_F23, OA_F33);
+  } else {
+    for (int o = 0; o < a.nout; o += 2) {
+      const bool two = o + 1 < a.nout;
+      T va[PTS], vb[PTS];
+      fill(a.code[o], va);
+      fill(two ? a.code[o + 1] : -1, vb);
+      X::fwd_pair(sm, [&](int q) { return mk<T>(va[q], vb[q]); }, a.dst[o], two ? a.dst[o + 1] : nullptr, line,
+                  lok, a.Kx, g, lw);
+    }
   }
 }
 
-template <class T, int NX, class X, bool MX>
+template <class T, int NX, class X, bool MX, class C>
 __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
-  HY_XLINE_PREAMBLE
+  HY_XLINE_PREAMBLE(C::D3)
+  constexpr int MN = C::MN, MK = C::MK;
   const int iy = lok ? (int)(line % a.ny) : 0, iz = lok ? (int)(line / a.ny) : 0;
   T dn[PTS], gmx[PTS], gmy[PTS], gmz[PTS];
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { gmy[p] = 0; gmz[p] = 0; }
-  X::inv_pair(sm, &a.n, &a.mux, nlines, line, lok, a.Kx, a.dkx, g, lw, dn, gmx);
-  if (nd >= 2) X::inv_pair(sm, &a.muy, nd >= 3 ? &a.muz : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, gmy, gmz);
+  X::template inv_pair<MN, MK>(sm, &a.n, &a.mux, nlines, line, lok, a.Kx, a.dkx, g, lw, dn, gmx);
+  if (nd >= 2)
+    X::template inv_pair<MN, MN>(sm, &a.muy, nd >= 3 ? &a.muz : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, gmy,
+                                 gmz);
   // external force (external_potential.f90:65-93) at the stage-start time,
   // evaluated point by point
   const bool wave_f = a.vext_kind == HY_VEXT_MODE || a.vext_kind == HY_VEXT_DRIVEN;
@@ -672,8 +745,10 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
     T m0[PTS], m1[PTS], m2[PTS];
 #pragma unroll
     for (int p = 0; p < PTS; ++p) { m1[p] = 0; m2[p] = 0; }
-    X::inv_pair(sm, &a.m[0], nd >= 2 ? &a.m[1] : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, m0, m1);
-    if (nd >= 3) X::inv_pair(sm, &a.m[2], nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, m2, nullptr);
+    X::template inv_pair<MN, MN>(sm, &a.m[0], nd >= 2 ? &a.m[1] : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw,
+                                 m0, m1);
+    if (nd >= 3)
+      X::template inv_pair<MN, MN>(sm, &a.m[2], nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, m2, nullptr);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) {
       const double inv = fd::rcp_nr(a.mass * (a.n0 + dn[p]));
@@ -687,9 +762,10 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
     if (nd == 1) {
       X::inv_pair(sm, &a.px[c], &a.px[c + 1], nlines, line, lok, a.Kx, a.dkx, g, lw, xa, xb);
     } else {
-      X::inv_pair(sm, &a.px[c], &a.py[c], nlines, line, lok, a.Kx, a.dkx, g, lw, xa, ya);
-      X::inv_pair(sm, &a.px[c + 1], &a.py[c + 1], nlines, line, lok, a.Kx, a.dkx, g, lw, xb, yb);
-      if (nd >= 3) X::inv_pair(sm, &a.pz[c], &a.pz[c + 1], nlines, line, lok, a.Kx, a.dkx, g, lw, za, zb);
+      X::template inv_pair<MK, MN>(sm, &a.px[c], &a.py[c], nlines, line, lok, a.Kx, a.dkx, g, lw, xa, ya);
+      X::template inv_pair<MK, MN>(sm, &a.px[c + 1], &a.py[c + 1], nlines, line, lok, a.Kx, a.dkx, g, lw, xb, yb);
+      if (nd >= 3)
+        X::template inv_pair<MN, MN>(sm, &a.pz[c], &a.pz[c + 1], nlines, line, lok, a.Kx, a.dkx, g, lw, za, zb);
     }
     X::fwd_pair(
         sm,
@@ -706,7 +782,7 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
 // max |u_d| over a line (model.f90:171-172)
 template <class T, int NX, class X>
 __device__ __attribute__((always_inline)) inline void umax_body(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
-  HY_XLINE_PREAMBLE
+  HY_XLINE_PREAMBLE(false)
   T dn[PTS], m0[PTS], m1[PTS], m2[PTS];
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { m1[p] = 0; m2[p] = 0; }
@@ -731,23 +807,23 @@ __device__ __attribute__((always_inline)) inline void umax_body(const PhysArgs* 
 // A pointer that arrives inside a struct or is read from memory would lose
 // what the compiler knows about kernel parameters, and every field of the
 // block would then be loaded per lane instead of once per wave.
-template <class T, int NX, class X>
+template <class T, int NX, class X, class C>
 __global__ __launch_bounds__(64) void k_phaseA(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ tw) {
-  phaseA_body<T, NX, X>(pa, tw);
+  phaseA_body<T, NX, X, C>(pa, tw);
 }
-template <class T, int NX, class X>
+template <class T, int NX, class X, class C>
 __global__ __launch_bounds__(64) void k_phaseA_b(const PhysArgs* __restrict__ pas, const cpx<T>* __restrict__ tw) {
   if (gate_closed(&pas[blockIdx.z].ctl->halt)) return;
-  phaseA_body<T, NX, X>(pas + blockIdx.z, tw);
+  phaseA_body<T, NX, X, C>(pas + blockIdx.z, tw);
 }
-template <class T, int NX, class X, bool MX>
+template <class T, int NX, class X, bool MX, class C>
 __global__ __launch_bounds__(64) void k_phaseB(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ tw) {
-  phaseB_body<T, NX, X, MX>(pa, tw);
+  phaseB_body<T, NX, X, MX, C>(pa, tw);
 }
-template <class T, int NX, class X, bool MX>
+template <class T, int NX, class X, bool MX, class C>
 __global__ __launch_bounds__(64) void k_phaseB_b(const PhysArgs* __restrict__ pas, const cpx<T>* __restrict__ tw) {
   if (gate_closed(&pas[blockIdx.z].ctl->halt)) return;
-  phaseB_body<T, NX, X, MX>(pas + blockIdx.z, tw);
+  phaseB_body<T, NX, X, MX, C>(pas + blockIdx.z, tw);
 }
 template <class T, int NX, class X>
 __global__ __launch_bounds__(64) void k_umax(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ tw) {
@@ -1253,11 +1329,27 @@ static int inv_chain(Model& m, const std::vector<InvSpec>& fs, std::vector<XIn>&
   xin.clear();
   const int nd = g.ndim;
   if (g.nz > 1) {
+    // Fields that differ only in their y or x multiplier share one z pass
+    // (n and its y derivative; mu for its x and y derivatives; each stress
+    // component for its x and y derivatives).  A k = 0 offset only reaches
+    // the ky = 0 row, which an i ky multiplier zeroes, so such a field may
+    // also share the z pass of one with another offset.
     std::vector<PencilField> zf, yf;
+    std::vector<int> zid(fs.size());
     for (size_t i = 0; i < fs.size(); ++i) {
       const auto& f = fs[i];
-      zf.push_back({f.src, zslot(m, (int)i), f.mz, f.src64 ? 1 : ct64(m), f.k0off});
-      yf.push_back({zslot(m, (int)i), xslot(m, f.xslot), f.my, ct64(m), 0.0});
+      const int in64 = f.src64 ? 1 : ct64(m);
+      int u = -1;
+      for (size_t j = 0; g.ny > 1 && j < zf.size() && u < 0; ++j)
+        if (zf[j].in == f.src && zf[j].in_f64 == in64 && zf[j].mul == f.mz &&
+            (zf[j].k0_offset == f.k0off || f.my == MUL_IK))
+          u = (int)j;
+      if (u < 0) {
+        u = (int)zf.size();
+        zf.push_back({f.src, zslot(m, u), f.mz, in64, f.k0off});
+      }
+      zid[i] = u;
+      yf.push_back({zslot(m, u), xslot(m, f.xslot), f.my, ct64(m), 0.0});
     }
     if ((int)fs.size() > m.nzslots) return set_error(HY_ERR_INVALID, "internal: z slots");
     prof_begin(m, PC_ZINV, pencil_bytes(m, zf, (size_t)g.Kx * g.Ky, 1, g.nz, g.Kz));
@@ -1271,7 +1363,7 @@ static int inv_chain(Model& m, const std::vector<InvSpec>& fs, std::vector<XIn>&
     for (auto& f : fs) xin.push_back({xslot(m, f.xslot), 0, f.mx, 0.0});
     // ny == 1 with nz > 1: the z output is already (Kx, 1, nz) x-lines
     if (g.ny == 1)
-      for (size_t i = 0; i < fs.size(); ++i) xin[i].p = zslot(m, (int)i);
+      for (size_t i = 0; i < fs.size(); ++i) xin[i].p = zslot(m, zid[i]);
   } else if (g.ny > 1) {
     std::vector<PencilField> yf;
     for (auto& f : fs) yf.push_back({f.src, xslot(m, f.xslot), f.my, f.src64 ? 1 : ct64(m), f.k0off});
@@ -1325,25 +1417,51 @@ enum XK { XK_A = 0, XK_B = 1, XK_UMAX = 2 };
 // device argument slots: one per distinct argument set
 enum XSlot { XA_STAGE = 0, XA_RELAX = 1, XB_STAGE = 2, XU_DT = 3, XA_N = 4 };
 #define HY_FN(K) reinterpret_cast<const void*>(&K)
-template <class T, int NX, class X, bool MX>
-static int launch_xk_with(int which, const PhysArgs* a, int nlines, const cpx<T>* tw, hipStream_t s) {
+// cfg: the XFun form chosen by xk_cfg; the fixed forms are built for lines
+// of 32 points and more (the 3-D grids of the timed runs), shorter lines
+// always take the run-time form.  The batched twins follow the same choice.
+template <class T, int NX>
+static int launch_xk(int which, bool maxwell, int cfg, const PhysArgs* a, int nlines, const cpx<T>* tw,
+                     hipStream_t s) {
+  using X = XWave<T, NX>;
+  using CR = XCfg<XF_RUNTIME>;
   const dim3 grid((unsigned)((nlines + X::LPW - 1) / X::LPW));
-  if (which == XK_A) return launch_x(HY_FN((k_phaseA<T, NX, X>)), HY_FN((k_phaseA_b<T, NX, X>)), grid, s, a, tw);
-  if (which == XK_B) return launch_x(HY_FN((k_phaseB<T, NX, X, MX>)), HY_FN((k_phaseB_b<T, NX, X, MX>)), grid, s, a, tw);
+  if (which == XK_A) {
+    if constexpr (NX >= 32) {
+      if (cfg == XF_IDEAL)
+        return launch_x(HY_FN((k_phaseA<T, NX, X, XCfg<XF_IDEAL>>)), HY_FN((k_phaseA_b<T, NX, X, XCfg<XF_IDEAL>>)),
+                        grid, s, a, tw);
+    }
+    return launch_x(HY_FN((k_phaseA<T, NX, X, CR>)), HY_FN((k_phaseA_b<T, NX, X, CR>)), grid, s, a, tw);
+  }
+  if (which == XK_B) {
+    // phase B without the Maxwell advection code where the closure has no
+    // evolving stress (fewer registers).  The fixed 3-D form is taken in FP32
+    // only: in FP64 it needs more registers than the run-time form (138 against
+    // 126 VGPRs at 256 points, which drops a wave per SIMD).
+    if constexpr (NX >= 32 && sizeof(T) == 4) {
+      if (cfg != XF_RUNTIME) {
+        using C3 = XCfg<XF_ANY3>;
+        if (maxwell)
+          return launch_x(HY_FN((k_phaseB<T, NX, X, true, C3>)), HY_FN((k_phaseB_b<T, NX, X, true, C3>)), grid, s,
+                          a, tw);
+        return launch_x(HY_FN((k_phaseB<T, NX, X, false, C3>)), HY_FN((k_phaseB_b<T, NX, X, false, C3>)), grid, s,
+                        a, tw);
+      }
+    }
+    if (maxwell)
+      return launch_x(HY_FN((k_phaseB<T, NX, X, true, CR>)), HY_FN((k_phaseB_b<T, NX, X, true, CR>)), grid, s, a,
+                      tw);
+    return launch_x(HY_FN((k_phaseB<T, NX, X, false, CR>)), HY_FN((k_phaseB_b<T, NX, X, false, CR>)), grid, s, a,
+                    tw);
+  }
   return launch_x(HY_FN((k_umax<T, NX, X>)), HY_FN((k_umax_b<T, NX, X>)), grid, s, a, tw);
 }
-// phase B without the Maxwell advection code where the closure has no
-// evolving stress (fewer registers)
-template <class T, int NX>
-static int launch_xk(int which, bool maxwell, const PhysArgs* a, int nlines, const cpx<T>* tw, hipStream_t s) {
-  if (maxwell) return launch_xk_with<T, NX, XWave<T, NX>, true>(which, a, nlines, tw, s);
-  return launch_xk_with<T, NX, XWave<T, NX>, false>(which, a, nlines, tw, s);
-}
 template <class T>
-static int dispatch_xk(int which, const Model& m, const PhysArgs* a, const cpx<T>* tw) {
+static int dispatch_xk(int which, int cfg, const Model& m, const PhysArgs* a, const cpx<T>* tw) {
   const int nlines = m.g.ny * m.g.nz;
   switch (m.g.nx) {
-#define HY_XK(LL) case LL: HY_TRY((launch_xk<T, LL>(which, m.evolve != 0, a, nlines, tw, m.st))); break;
+#define HY_XK(LL) case LL: HY_TRY((launch_xk<T, LL>(which, m.evolve != 0, cfg, a, nlines, tw, m.st))); break;
     HY_XK(4) HY_XK(8) HY_XK(12) HY_XK(16) HY_XK(24) HY_XK(32) HY_XK(48) HY_XK(64) HY_XK(96)
     HY_XK(128) HY_XK(256)
 #undef HY_XK
@@ -1352,6 +1470,29 @@ static int dispatch_xk(int which, const Model& m, const PhysArgs* a, const cpx<T
   HY_HIP(hipGetLastError());
   HY_TRACE(which == XK_A ? "x_phaseA" : (which == XK_B ? "x_phaseB" : "x_umax"), m.st);
   return HY_OK;
+}
+// The kernel form for an argument set (XFun): a fixed form only where every
+// input the kernel reads has the layout and multiplier that form assumes and,
+// for phase A, the functional terms and output list are the form's own.
+static int xk_cfg(const Model& m, int which, const PhysArgs& a) {
+  const Grid& g = m.g;
+  if (which == XK_UMAX || g.ndim != 3 || g.ny < 2 || g.nz < 2 || a.nd != 3) return XF_RUNTIME;
+  bool ok = true;
+  auto in = [&](const XIn& x, int mul) { ok = ok && x.p && !x.is64 && x.k0off == 0.0 && x.mul == mul; };
+  in(a.n, XM_NONE);
+  if (which == XK_A) {
+    for (int d = 0; d < 3; ++d) in(a.m[d], XM_NONE);
+    if (!(a.ideal && !a.tf && !a.grad) || a.nout != 10) return XF_RUNTIME;
+    static const int order[10] = {OA_MU, OA_U1, OA_U2, OA_U3, OA_F11, OA_F12, OA_F13, OA_F22, OA_F23, OA_F33};
+    for (int o = 0; o < 10; ++o) ok = ok && a.code[o] == order[o];
+    return ok ? XF_IDEAL : XF_RUNTIME;
+  }
+  in(a.mux, XM_IK); in(a.muy, XM_NONE); in(a.muz, XM_NONE);
+  if (a.evolve) {
+    for (int d = 0; d < 3; ++d) in(a.m[d], XM_NONE);
+    for (int c = 0; c < 6; ++c) { in(a.px[c], XM_IK); in(a.py[c], XM_NONE); in(a.pz[c], XM_NONE); }
+  }
+  return ok ? XF_ANY3 : XF_RUNTIME;
 }
 // Compulsory bytes of one x-kernel launch: every distinct input x-line the
 // kernel reads once, every output written once.
@@ -1431,7 +1572,8 @@ static int run_xk(Model& m, int which, int slot, const PhysArgs& a) {
   const PhysArgs* da = nullptr;
   HY_TRY(xargs_dev(m, slot, a, &da));
   prof_begin(m, which == XK_A ? PC_XA : (which == XK_B ? PC_XB : PC_UMAX), xk_bytes(m, which, a));
-  int rc = mixed(m) ? dispatch_xk<float>(which, m, da, t.d32) : dispatch_xk<double>(which, m, da, t.d64);
+  const int cfg = xk_cfg(m, which, a);
+  int rc = mixed(m) ? dispatch_xk<float>(which, cfg, m, da, t.d32) : dispatch_xk<double>(which, cfg, m, da, t.d64);
   prof_end(m);
   return rc;
 }
