@@ -191,13 +191,15 @@ __host__ __device__ inline double alpha_from_ip12_from(const double* tab, double
   return alpha;
 }
 
-// ---------------------------------------------------------------- device batch
+// ---------------------------------------------------------------- field kernels
+// The forms the field kernels evaluate (also compiled for the host, where
+// hy_fd_eval exposes them for checks against the functions above).
 // Device-side table access: buffer loads through a wave-uniform descriptor
 // over the whole table with 32-bit byte offsets.  (Plain loads from the table
 // pointer become generic-address loads, whose waits also drain LDS traffic.)
 // node() keeps every offset inside the table; the descriptor's record count
 // bounds it as well (out-of-range buffer loads return 0).
-__device__ inline double tabd(const double* tab, int idx) {
+__host__ __device__ inline double tabd(const double* tab, int idx) {
 #if defined(__HIP_DEVICE_COMPILE__)
   const __amdgpu_buffer_rsrc_t r =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(tab), (short)0, NTAB * NC * 8, 0x00020000);
@@ -207,11 +209,67 @@ __device__ inline double tabd(const double* tab, int idx) {
 #endif
 }
 
+// ---- the expansions outside the table, inline for the field kernels.
+// The same series as sommerfeld() and classical(), with the half-integer
+// powers formed from one square root and integer powers instead of pow():
+// they agree to a few ulp.  Kept inline because a call from a kernel makes
+// the compiler reserve registers for the calling convention across the
+// whole kernel (the electron x-line kernel needed about 30 more VGPRs).
+// Sommerfeld (a > AMAX): I_p^(m) = sum_j c_j/(p+1) prod_{l<m}(e_j - l) a^(e_j - m),
+// e_j = p + 1 - 2 j, c = {1, pi^2/6 p(p+1), 7 pi^4/360 (p+1)p(p-1)(p-2)}.
+constexpr double SOM_C1M = (PI * PI / 6.0) * (-0.5) * 0.5;                          // p = -1/2
+constexpr double SOM_C2M = (7.0 * PI * PI * PI * PI / 360.0) * 0.5 * (-0.5) * (-1.5) * (-2.5);
+constexpr double SOM_C1P = (PI * PI / 6.0) * 0.5 * 1.5;                             // p = +1/2
+constexpr double SOM_C2P = (7.0 * PI * PI * PI * PI / 360.0) * 1.5 * 0.5 * (-0.5) * (-1.5);
+// classical (a < AMIN): I_p^(m) = Gamma(p+1) sum_{j=1..12} (-1)^(j+1) e^(j a) j^(m-p-1)
+__host__ __device__ __attribute__((always_inline)) inline void tail_ip12_im12(double a, double& ip, double& im) {
+  if (a > AMAX) {
+    const double r = sqrt(a), ia = 1.0 / a, ia2 = ia * ia;
+    // p = 1/2: e = 3/2, -1/2, -5/2; p = -1/2: e = 1/2, -3/2, -7/2 (coefficient c_j / (p+1))
+    ip = (r * a) / 1.5 + (SOM_C1P / 1.5) * (r * ia) + (SOM_C2P / 1.5) * (r * ia2 * ia);
+    im = r / 0.5 + (SOM_C1M / 0.5) * (r * ia2) + (SOM_C2M / 0.5) * (r * ia2 * ia2);
+    return;
+  }
+  double sp = 0.0, sm = 0.0, sgn = 1.0;
+#pragma unroll 1
+  for (int j = 1; j <= 12; ++j) {
+    const double e = exp(j * a), rj = sqrt((double)j);
+    sp += sgn * e / (j * rj);  // j^(-3/2)
+    sm += sgn * e / rj;        // j^(-1/2)
+    sgn = -sgn;
+  }
+  ip = sp * (0.5 * SQRTPI);
+  im = sm * SQRTPI;
+}
+__host__ __device__ __attribute__((always_inline)) inline void tail_im12_d1_d2(double a, double& im, double& d1, double& d2) {
+  if (a > AMAX) {
+    const double r = sqrt(a), ia = 1.0 / a, ia2 = ia * ia;
+    // p = -1/2, e = 1/2, -3/2, -7/2; m-th derivative: prod_{l<m} (e - l) a^(e - m)
+    const double c0 = 1.0 / 0.5, c1 = SOM_C1M / 0.5, c2 = SOM_C2M / 0.5;
+    im = c0 * r + c1 * (r * ia2) + c2 * (r * ia2 * ia2);
+    d1 = c0 * 0.5 * (r * ia) + c1 * (-1.5) * (r * ia2 * ia) + c2 * (-3.5) * (r * ia2 * ia2 * ia);
+    d2 = c0 * (0.5 * -0.5) * (r * ia2) + c1 * (-1.5 * -2.5) * (r * ia2 * ia2) +
+         c2 * (-3.5 * -4.5) * (r * ia2 * ia2 * ia2);
+    return;
+  }
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, sgn = 1.0;
+#pragma unroll 1
+  for (int j = 1; j <= 12; ++j) {
+    const double e = sgn * exp(j * a), rj = sqrt((double)j);
+    s0 += e / rj;      // j^(-1/2)
+    s1 += e * rj;      // j^(1/2)
+    s2 += e * (j * rj);  // j^(3/2)
+    sgn = -sgn;
+  }
+  im = s0 * SQRTPI;
+  d1 = s1 * SQRTPI;
+  d2 = s2 * SQRTPI;
+}
+
 // ip12_im12 with device gathers
-__device__ inline void ip12_im12_dev(const double* tab, double a, double& ip, double& im) {
+__host__ __device__ inline void ip12_im12_dev(const double* tab, double a, double& ip, double& im) {
   if (a > AMAX || a < AMIN) {
-    ip = ip12(tab, a);
-    im = im12(tab, a);
+    tail_ip12_im12(a, ip, im);
     return;
   }
   const int i = node(a);
@@ -227,11 +285,9 @@ __device__ inline void ip12_im12_dev(const double* tab, double a, double& ip, do
 }
 
 // im12_d1_d2 with device gathers
-__device__ inline void im12_d1_d2_dev(const double* tab, double a, double& im, double& d1, double& d2) {
+__host__ __device__ inline void im12_d1_d2_dev(const double* tab, double a, double& im, double& d1, double& d2) {
   if (a > AMAX || a < AMIN) {
-    im = im12(tab, a);
-    d1 = dim12(tab, a);
-    d2 = d2im12(tab, a);
+    tail_im12_d1_d2(a, im, d1, d2);
     return;
   }
   const int i = node(a);
@@ -249,7 +305,7 @@ __device__ inline void im12_d1_d2_dev(const double* tab, double a, double& im, d
 }
 
 // 1/d from the hardware reciprocal and one Newton-Raphson correction
-__device__ inline double rcp_nr(double d) {
+__host__ __device__ inline double rcp_nr(double d) {
 #if defined(__HIP_DEVICE_COMPILE__)
   const double r = __builtin_amdgcn_rcp(d);
   return fma(fma(-d, r, 1.0), r, r);
@@ -261,7 +317,7 @@ __device__ inline double rcp_nr(double d) {
 // The I_{1/2} interpolant at a inside the table and its first two
 // alpha-derivatives: the exact derivatives of the cubic piece of node i
 // (the piece the Newton iteration solves on).
-__device__ inline void ip12_d2_dev(const double* tab, double a, int i, double& ip, double& dp, double& d2p) {
+__host__ __device__ inline void ip12_d2_dev(const double* tab, double a, int i, double& ip, double& dp, double& d2p) {
   const int o = i * NC;
   const double t = (a - (AMIN + i * H)) * INVH;
   const double t2 = t * t, t3 = t2 * t;
@@ -290,7 +346,7 @@ __device__ inline void ip12_d2_dev(const double* tab, double a, int i, double& i
 // |I''/(2 I')| step^2 <= 2e-16 max(1, |alpha|)^2 (|I''/(2 I')| <= 1/2).
 // From the field kernels' second-order start this is one table evaluation
 // per point at the density excursions of a run (|dn/n0| of a few 1e-3).
-__device__ inline double alpha_from_ip12_dev(const double* tab, double y, double a_start) {
+__host__ __device__ inline double alpha_from_ip12_dev(const double* tab, double y, double a_start) {
   const double huge = 1.7976931348623157e308;
   double alpha = a_start;
   if (!(alpha > AMAX || alpha < AMIN)) {

@@ -808,11 +808,11 @@ __device__ __attribute__((always_inline)) inline void umax_body(const PhysArgs* 
 // what the compiler knows about kernel parameters, and every field of the
 // block would then be loaded per lane instead of once per wave.
 template <class T, int NX, class X, class C>
-__global__ __launch_bounds__(64) void k_phaseA(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ tw) {
+__global__ __launch_bounds__(64, sizeof(T) == 4 ? 5 : 1) void k_phaseA(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ tw) {
   phaseA_body<T, NX, X, C>(pa, tw);
 }
 template <class T, int NX, class X, class C>
-__global__ __launch_bounds__(64) void k_phaseA_b(const PhysArgs* __restrict__ pas, const cpx<T>* __restrict__ tw) {
+__global__ __launch_bounds__(64, sizeof(T) == 4 ? 5 : 1) void k_phaseA_b(const PhysArgs* __restrict__ pas, const cpx<T>* __restrict__ tw) {
   if (gate_closed(&pas[blockIdx.z].ctl->halt)) return;
   phaseA_body<T, NX, X, C>(pas + blockIdx.z, tw);
 }
@@ -3231,6 +3231,27 @@ int hy_fd_eval(int which, int n, const double* x, double* y) {
         if (!(a > 0.0)) return set_error(HY_ERR_INVALID, "fermi_dirac: I_{1/2} inverse needs y > 0");
         y[i] = fd::alpha_from_ip12(tab, a);
         break;
+      // the field kernels' forms (table gathers, inline expansions outside it)
+      case 8: case 9: case 10: {
+        double im, d1, d2;
+        fd::im12_d1_d2_dev(tab, a, im, d1, d2);
+        y[i] = which == 8 ? im : (which == 9 ? d1 : d2);
+        break;
+      }
+      case 11: {
+        double ip, im;
+        fd::ip12_im12_dev(tab, a, ip, im);
+        y[i] = ip;
+        break;
+      }
+      case 12: {
+        if (!(a > 0.0)) return set_error(HY_ERR_INVALID, "fermi_dirac: I_{1/2} inverse needs y > 0");
+        // started from the reference's closed-form guess (fermi_dirac.f90:246-248)
+        double a0 = std::fmax(std::log(2.0 * a / fd::SQRTPI), std::pow(1.5 * a, 2.0 / 3.0) - 1.0);
+        if (a < 1.0) a0 = std::log(2.0 * a / fd::SQRTPI);
+        y[i] = fd::alpha_from_ip12_dev(tab, a, a0);
+        break;
+      }
       default: return set_error(HY_ERR_INVALID, "hy_fd_eval: unknown function");
     }
   }

@@ -104,7 +104,9 @@ int hy_spec_mean_noalloc(hy_grid g, const double* d_u, double* h_mean, void* str
 
 /* ---- fermi_dirac.f90 functions (host): which = 0 I_{-1/2}, 1 I_{1/2},
    2 I_{3/2}, 3 I'_{-1/2}, 4 I''_{-1/2}, 5 xi, 6 dxi/dalpha, 7 inverse of I_{1/2}
-   (fermi_dirac.f90:150-267) */
+   (fermi_dirac.f90:150-267); 8-12 the forms the field kernels evaluate,
+   computed on the host from the same source: 8 I_{-1/2}, 9 I'_{-1/2},
+   10 I''_{-1/2}, 11 I_{1/2}, 12 inverse of I_{1/2} */
 int hy_fd_eval(int which, int n, const double* h_x, double* h_y);
 
 /* ---- stress_t (src/closure/stress.f90:60-105), code-unit eta, xi -------- */

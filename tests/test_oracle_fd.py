@@ -108,3 +108,23 @@ def test_golden_known_answers():
         tol = g["inverse"]["rel_tol"] * max(1.0, abs(a))
         assert abs(v(fd.alpha_from_ip12, y) - a) < tol
         assert abs(float(fd_eval(FD_ALPHA, y)[0]) - a) < tol
+
+
+def test_field_kernel_forms_match_reference_forms():
+    """The Fermi-Dirac code the field kernels run (table gathers; inline
+    Sommerfeld and classical expansions outside [-30, 300] with powers from
+    one square root; the Halley-started inverse) against the reference forms
+    (fermi_dirac.f90:150-267, pow-based tails), compiled from the same source
+    for the host (hy_fd_eval 8-12).  Covers both tails and the table."""
+    from hydft_amd.model import fd_eval
+    xs = np.concatenate([np.linspace(-80.0, -30.05, 57), np.linspace(-30.0, 300.0, 661),
+                         np.linspace(300.05, 5000.0, 53), [-30.0, 300.0, -29.999, 299.999]])
+    for kernel, ref in ((8, 0), (9, 3), (10, 4), (11, 1)):
+        got, want = fd_eval(kernel, xs), fd_eval(ref, xs)
+        np.testing.assert_allclose(got, want, rtol=2e-14, atol=0, err_msg=f"form {kernel} vs {ref}")
+    # inverse: y over the table and both tails, each back to the reference inverse
+    ys = fd_eval(1, np.concatenate([np.linspace(-45.0, -30.5, 8), np.linspace(-29.0, 299.0, 200),
+                                    np.linspace(301.0, 900.0, 8)]))
+    a_ref = fd_eval(7, ys)
+    a_ker = fd_eval(12, ys)
+    np.testing.assert_allclose(a_ker, a_ref, rtol=1e-13, atol=1e-13)
