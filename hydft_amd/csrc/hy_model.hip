@@ -525,6 +525,7 @@ struct PhysArgs {
   int vext_kind, vext_td;
   double amp, ramp, omega, kvec[3];
   const double* fext;  // gaussian: 3 real fields
+  const double* vsc;   // mode / driven: sin(kx x_i), then cos(kx x_i), i < nx
   double* red;         // per-line partial sums
   // outputs: phase A in the order of code[]; phase B: H_1..H_nd, then ADV_1..6
   int nout;
@@ -536,18 +537,21 @@ struct PhysArgs {
 // every choice from the argument block.  The other forms are for 3-D grids,
 // whose x-line inputs are all compute-type slots with no k = 0 offset and a
 // multiplier fixed by their role (XMul): XF_ANY3 fixes that layout (phase B
-// of any model); XF_IDEAL also fixes phase A of a classical ion model (the
+// of any model), XF_NOF3 also the absence of an external force (phase B
+// without the force code: 102 against 73 VGPRs at 256 points in FP32);
+// XF_IDEAL also fixes phase A of a classical ion model (the
 // ideal term as the only pointwise functional; mu, u and the six fluxes
 // emitted in the order the host lists them).  Fixing these at compile time
 // removes the uniform branches on the inputs' types and on the output codes,
 // with the register moves at their joins.  (A fixed form of the electron
 // functional was tried: its straight-line code needed more than the 128
 // VGPRs of 4 waves per SIMD and spilled.)
-enum XFun { XF_RUNTIME = 0, XF_IDEAL = 1, XF_ANY3 = 2 };
+enum XFun { XF_RUNTIME = 0, XF_IDEAL = 1, XF_ANY3 = 2, XF_NOF3 = 3 };
 template <int F>
 struct XCfg {
   static constexpr bool D3 = F != XF_RUNTIME;
   static constexpr bool IDEAL = F == XF_IDEAL;
+  static constexpr bool NOFORCE = F == XF_NOF3;
   static constexpr int MN = D3 ? XM_NONE : -1, MK = D3 ? XM_IK : -1;
 };
 
@@ -696,15 +700,18 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
     X::template inv_pair<MN, MN>(sm, &a.muy, nd >= 3 ? &a.muz : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, gmy,
                                  gmz);
   // external force (external_potential.f90:65-93) at the stage-start time,
-  // evaluated point by point
-  const bool wave_f = a.vext_kind == HY_VEXT_MODE || a.vext_kind == HY_VEXT_DRIVEN;
-  const bool gauss_f = a.vext_kind == HY_VEXT_GAUSSIAN && lok;
-  double amp = 0.0, t = 0.0, yz = 0.0;
+  // evaluated point by point.  The wave force amp k sin(kx x + ph), with the
+  // line's phase ph = ky y + kz z - w t, is formed as
+  // amp k (sin(kx x) cos(ph) + cos(kx x) sin(ph)) from the model's table of
+  // sin(kx x), cos(kx x): one sincos per lane instead of one sin per point.
+  const bool wave_f = !C::NOFORCE && (a.vext_kind == HY_VEXT_MODE || a.vext_kind == HY_VEXT_DRIVEN);
+  const bool gauss_f = !C::NOFORCE && a.vext_kind == HY_VEXT_GAUSSIAN && lok;
+  double amp = 0.0, sph = 0.0, cph = 0.0;
   if (wave_f) {
-    t = a.vext_td ? a.ctl->ts : 0.0;
+    const double t = a.vext_td ? a.ctl->ts : 0.0;
     amp = a.amp;
     if (a.vext_td && a.ramp > 0.0) amp = amp * fmin(1.0, t / a.ramp);
-    yz = a.kvec[1] * (iy * a.dy) + a.kvec[2] * (iz * a.dz);
+    sincos(a.kvec[1] * (iy * a.dy) + a.kvec[2] * (iz * a.dz) - a.omega * t, &sph, &cph);
   }
   const size_t nr = (size_t)NX * a.ny * a.nz;
   const double* fe = a.fext + (lok ? line : 0) * NX;
@@ -716,7 +723,8 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
     const bool live = lok && x >= 0;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     if (wave_f) {
-      const double sv = amp * sin(a.kvec[0] * (x * a.dx) + yz - a.omega * t);
+      const int xi = x >= 0 ? x : 0;
+      const double sv = amp * (gld<double>(a.vsc, xi) * cph + gld<double>(a.vsc, NX + xi) * sph);
       f0 = sv * a.kvec[0]; f1 = sv * a.kvec[1]; f2 = sv * a.kvec[2];
     } else if (gauss_f && x >= 0) {
       f0 = fe[x]; f1 = fe[nr + x]; f2 = fe[2 * nr + x];
@@ -727,7 +735,7 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
     h2[p] = (T)(n * (f2 - gmz[p]));
     if (live) { s0 += n * f0; s1 += n * f1; s2 += n * f2; }
   }
-  if (a.vext_kind != HY_VEXT_NONE) {
+  if (!C::NOFORCE && a.vext_kind != HY_VEXT_NONE) {
     s0 = X::line_sum(s0); s1 = X::line_sum(s1); s2 = X::line_sum(s2);
     if (g == 0 && lok) { a.red[line * 3] = s0; a.red[line * 3 + 1] = s1; a.red[line * 3 + 2] = s2; }
   }
@@ -1196,6 +1204,7 @@ struct Model {
   c64* uold = nullptr;  // 3 * nt
   double* kcomb = nullptr;
   double* fext = nullptr;
+  double* vsc = nullptr;  // wave force: sin, cos of kx x at the nx grid points
   double* fdtab = nullptr;
   void* xb = nullptr;   // x-line slots (compute type), each Kx*ny*nz
   void* zb = nullptr;   // z-pass outputs, each Kx*Ky*nz
@@ -1440,8 +1449,16 @@ static int launch_xk(int which, bool maxwell, int cfg, const PhysArgs* a, int nl
     // only: in FP64 it needs more registers than the run-time form (138 against
     // 126 VGPRs at 256 points, which drops a wave per SIMD).
     if constexpr (NX >= 32 && sizeof(T) == 4) {
-      if (cfg != XF_RUNTIME) {
+      if (cfg == XF_ANY3) {
         using C3 = XCfg<XF_ANY3>;
+        if (maxwell)
+          return launch_x(HY_FN((k_phaseB<T, NX, X, true, C3>)), HY_FN((k_phaseB_b<T, NX, X, true, C3>)), grid, s,
+                          a, tw);
+        return launch_x(HY_FN((k_phaseB<T, NX, X, false, C3>)), HY_FN((k_phaseB_b<T, NX, X, false, C3>)), grid, s,
+                        a, tw);
+      }
+      if (cfg == XF_NOF3) {
+        using C3 = XCfg<XF_NOF3>;
         if (maxwell)
           return launch_x(HY_FN((k_phaseB<T, NX, X, true, C3>)), HY_FN((k_phaseB_b<T, NX, X, true, C3>)), grid, s,
                           a, tw);
@@ -1492,7 +1509,8 @@ static int xk_cfg(const Model& m, int which, const PhysArgs& a) {
     for (int d = 0; d < 3; ++d) in(a.m[d], XM_NONE);
     for (int c = 0; c < 6; ++c) { in(a.px[c], XM_IK); in(a.py[c], XM_NONE); in(a.pz[c], XM_NONE); }
   }
-  return ok ? XF_ANY3 : XF_RUNTIME;
+  if (!ok) return XF_RUNTIME;
+  return a.vext_kind == HY_VEXT_NONE ? XF_NOF3 : XF_ANY3;
 }
 // Compulsory bytes of one x-kernel launch: every distinct input x-line the
 // kernel reads once, every output written once.
@@ -1592,7 +1610,7 @@ static PhysArgs phys_base(Model& m) {
   a.fdtab = m.fdtab; a.ctl = m.ctl; a.evolve = m.evolve;
   a.vext_kind = m.vext_kind; a.vext_td = m.vext_td; a.amp = m.vamp; a.ramp = m.vramp;
   a.omega = m.vomega; a.kvec[0] = m.kvec[0]; a.kvec[1] = m.kvec[1]; a.kvec[2] = m.kvec[2];
-  a.fext = m.fext; a.red = m.red;
+  a.fext = m.fext; a.vsc = m.vsc; a.red = m.red;
   return a;
 }
 
@@ -1898,7 +1916,7 @@ static double fun_kernel(const Model& m, double k) {
 }
 
 static void model_free(Model& m) {
-  for (void* p : {(void*)m.S, (void*)m.R, (void*)m.uold, (void*)m.kcomb, (void*)m.fext, (void*)m.fdtab,
+  for (void* p : {(void*)m.S, (void*)m.R, (void*)m.uold, (void*)m.kcomb, (void*)m.fext, (void*)m.vsc, (void*)m.fdtab,
                   m.xb, m.zb, m.fb, (void*)m.red, (void*)m.ctl, (void*)m.probe, (void*)m.aprobe, (void*)m.d_xargs,
                   (void*)m.d_real, (void*)m.d_full})
     if (p) (void)hipFree(p);
@@ -2335,6 +2353,18 @@ int model_build(Model& m, const hy_model_desc* d) {
   if (g.nz > 1) m.dxmin = std::min(m.dxmin, g.dz);
   HY_TRY(dmalloc((void**)&m.kcomb, sizeof(double) * m.nt));
   HY_HIP(hipMemcpy(m.kcomb, kc.data(), sizeof(double) * m.nt, hipMemcpyHostToDevice));
+  // wave force: the x factor of sin(kx x + ky y + kz z - w t) per grid point,
+  // as sin(kx x) and cos(kx x) (the x-line kernels add the line's phase)
+  if (m.vext_kind == HY_VEXT_MODE || m.vext_kind == HY_VEXT_DRIVEN) {
+    std::vector<double> sc(2 * (size_t)g.nx);
+    for (int i = 0; i < g.nx; ++i) {
+      const double ax = m.kvec[0] * (i * g.dx);
+      sc[i] = std::sin(ax);
+      sc[g.nx + i] = std::cos(ax);
+    }
+    HY_TRY(dmalloc((void**)&m.vsc, sizeof(double) * sc.size()));
+    HY_HIP(hipMemcpy(m.vsc, sc.data(), sizeof(double) * sc.size(), hipMemcpyHostToDevice));
+  }
   // static gaussian force -grad(v) (external_potential.f90:84-91)
   if (m.vext_kind == HY_VEXT_GAUSSIAN) {
     std::vector<double> v(g.nreal());
