@@ -505,6 +505,13 @@ enum OutB { OB_H1 = 0, OB_H2, OB_H3, OB_ADV1 };  // ADV1..ADV6 = 3..8
 // kernel argument whose arrays are indexed at run time, or whose members have
 // their address taken, is copied whole into per-lane scratch memory in every
 // workgroup.
+// Density table of the mixed electron functional: RT_N intervals over
+// |n - n0| <= RT_R n0.  4-point Lagrange interpolation on it reproduces the
+// per-point Fermi-Dirac evaluation (table, Halley and Newton) to about
+// 1e-13 relative for alpha0 in [-8, 60]; points outside take that evaluation.
+constexpr int RT_N = 2048;
+constexpr double RT_R = 0.2;
+
 struct PhysArgs {
   int nd, Kx, ny, nz;
   double dx, dy, dz;
@@ -512,6 +519,11 @@ struct PhysArgs {
   double n0, mass, beta, cn, alpha0, cg;
   double inv_n0, inv_beta, inv_cn;  // reciprocals: the per-point functional multiplies
   double dadn, d2adn2;  // d alpha / d n, d2 alpha / d n2 at n0: start of the per-point Newton iteration
+  // density table (mixed precision only, null otherwise): at x_i = -rt_r + i h
+  // (x = (n - n0) / n0, i <= RT_N) the node holds {alpha - alpha0, xi,
+  // d xi / d n, 0}; rt_in bounds |x| where the 4-point stencil is inside
+  const double* rtab;
+  double rt_r, rt_invh, rt_in;
   int ideal, tf, grad;
   const double* fdtab;
   Ctl* ctl;
@@ -616,19 +628,41 @@ __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs
     // only grad(mu) enters the momentum equation (model.f90:99).
     double v = ideal ? log1p((double)dn[p] * a.inv_n0) * a.inv_beta : 0.0;  // fe_ideal_classical.f90:40
     if (tf || grad) {
-      const double al = live && n > 0.0
-                            ? fd::alpha_from_ip12_dev(a.fdtab, n * a.inv_cn,
-                                                      a.alpha0 + (double)dn[p] * (a.dadn + 0.5 * a.d2adn2 * (double)dn[p]))
-                            : a.alpha0;
-      if (tf) v += (al - a.alpha0) * a.inv_beta;  // fe_kinetic_tfk.f90:80-81
-      if (grad) {                                 // fe_kinetic_tfk.f90:130-137
-        double im, d1, d2;
-        fd::im12_d1_d2_dev(a.fdtab, al, im, d1, d2);
-        const double rim = fd::rcp_nr(im);
-        const double xi = d1 * rim * rim;
-        const double dxi = (d2 * im - 2.0 * d1 * d1) * (rim * rim * rim);
-        const double dxidn = dxi * rim * (2.0 * a.inv_cn);
-        v -= a.cg * (dxidn * (double)g2[p] + 2.0 * xi * (double)lap[p]);
+      const double x = (double)dn[p] * a.inv_n0;
+      if (a.rtab && fabs(x) < a.rt_in) {
+        // mixed precision: the node values around x, 4-point Lagrange weights
+        const double u = (x + a.rt_r) * a.rt_invh;
+        const int i = (int)u - 1;  // u >= 2 here, so the stencil is i..i+3, inside the table
+        const double t = u - (double)(i + 1);
+        const double tm1 = t - 1.0, tm2 = t - 2.0, tp1 = t + 1.0;
+        const double w0 = -t * tm1 * tm2 * (1.0 / 6.0), w1 = tp1 * tm1 * tm2 * 0.5;
+        const double w2 = -tp1 * t * tm2 * 0.5, w3 = tp1 * t * tm1 * (1.0 / 6.0);
+        double r[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const double w = k == 0 ? w0 : (k == 1 ? w1 : (k == 2 ? w2 : w3));
+          const double* q = a.rtab + 4 * (i + k);
+          r[0] += w * gld<double>(q, 0);
+          r[1] += w * gld<double>(q, 1);
+          r[2] += w * gld<double>(q, 2);
+        }
+        if (tf) v += r[0] * a.inv_beta;
+        if (grad) v -= a.cg * (r[2] * (double)g2[p] + 2.0 * r[1] * (double)lap[p]);
+      } else {
+        const double al = live && n > 0.0
+                              ? fd::alpha_from_ip12_dev(a.fdtab, n * a.inv_cn,
+                                                        a.alpha0 + (double)dn[p] * (a.dadn + 0.5 * a.d2adn2 * (double)dn[p]))
+                              : a.alpha0;
+        if (tf) v += (al - a.alpha0) * a.inv_beta;  // fe_kinetic_tfk.f90:80-81
+        if (grad) {                                 // fe_kinetic_tfk.f90:130-137
+          double im, d1, d2;
+          fd::im12_d1_d2_dev(a.fdtab, al, im, d1, d2);
+          const double rim = fd::rcp_nr(im);
+          const double xi = d1 * rim * rim;
+          const double dxi = (d2 * im - 2.0 * d1 * d1) * (rim * rim * rim);
+          const double dxidn = dxi * rim * (2.0 * a.inv_cn);
+          v -= a.cg * (dxidn * (double)g2[p] + 2.0 * xi * (double)lap[p]);
+        }
       }
     }
     mu[p] = (T)v;
@@ -1205,6 +1239,7 @@ struct Model {
   double* kcomb = nullptr;
   double* fext = nullptr;
   double* vsc = nullptr;  // wave force: sin, cos of kx x at the nx grid points
+  double* rtab = nullptr;  // mixed electrons: the functional's factors on a density grid
   double* fdtab = nullptr;
   void* xb = nullptr;   // x-line slots (compute type), each Kx*ny*nz
   void* zb = nullptr;   // z-pass outputs, each Kx*Ky*nz
@@ -1608,6 +1643,10 @@ static PhysArgs phys_base(Model& m) {
   a.d2adn2 = m.d2adn2;
   a.ideal = m.ideal; a.tf = m.tf; a.grad = m.grad;
   a.fdtab = m.fdtab; a.ctl = m.ctl; a.evolve = m.evolve;
+  a.rtab = m.rtab;
+  a.rt_r = RT_R;
+  a.rt_invh = RT_N / (2.0 * RT_R);
+  a.rt_in = RT_R - 2.0 * (2.0 * RT_R / RT_N);
   a.vext_kind = m.vext_kind; a.vext_td = m.vext_td; a.amp = m.vamp; a.ramp = m.vramp;
   a.omega = m.vomega; a.kvec[0] = m.kvec[0]; a.kvec[1] = m.kvec[1]; a.kvec[2] = m.kvec[2];
   a.fext = m.fext; a.vsc = m.vsc; a.red = m.red;
@@ -1916,7 +1955,8 @@ static double fun_kernel(const Model& m, double k) {
 }
 
 static void model_free(Model& m) {
-  for (void* p : {(void*)m.S, (void*)m.R, (void*)m.uold, (void*)m.kcomb, (void*)m.fext, (void*)m.vsc, (void*)m.fdtab,
+  for (void* p : {(void*)m.S, (void*)m.R, (void*)m.uold, (void*)m.kcomb, (void*)m.fext, (void*)m.vsc, (void*)m.rtab,
+                  (void*)m.fdtab,
                   m.xb, m.zb, m.fb, (void*)m.red, (void*)m.ctl, (void*)m.probe, (void*)m.aprobe, (void*)m.d_xargs,
                   (void*)m.d_real, (void*)m.d_full})
     if (p) (void)hipFree(p);
@@ -2316,6 +2356,22 @@ int model_build(Model& m, const hy_model_desc* d) {
     HY_HIP(hipMemcpy(m.fdtab, tab, sizeof(double) * fd::NTAB * fd::NC, hipMemcpyHostToDevice));
     m.dadn = 2.0 / (m.cn * fd::im12(tab, m.alpha0));
     m.d2adn2 = -m.dadn * m.dadn * fd::dim12(tab, m.alpha0) / fd::im12(tab, m.alpha0);
+    if (mixed(m)) {
+      // the reference's evaluation at each node (fermi_dirac.f90:241-267, fe_kinetic_tfk.f90:130-137)
+      std::vector<double> rt(4 * (size_t)(RT_N + 1), 0.0);
+      const double h = 2.0 * RT_R / RT_N;
+      for (int i = 0; i <= RT_N; ++i) {
+        const double n = m.n0 * (1.0 + (-RT_R + i * h));
+        const double al = fd::alpha_from_ip12(tab, n / m.cn);
+        double im, d1, d2;
+        fd::im12_d1_d2(tab, al, im, d1, d2);
+        rt[4 * i + 0] = al - m.alpha0;
+        rt[4 * i + 1] = d1 / (im * im);
+        rt[4 * i + 2] = (d2 * im - 2.0 * d1 * d1) / (im * im * im) / im * (2.0 / m.cn);
+      }
+      HY_TRY(dmalloc((void**)&m.rtab, sizeof(double) * rt.size()));
+      HY_HIP(hipMemcpy(m.rtab, rt.data(), sizeof(double) * rt.size(), hipMemcpyHostToDevice));
+    }
   }
   HY_TRY(dmalloc((void**)&m.d_xargs, sizeof(PhysArgs) * XA_N));
   if (hipHostMalloc((void**)&m.p_xargs, sizeof(PhysArgs) * XA_N, hipHostMallocDefault) != hipSuccess) {
@@ -2742,6 +2798,8 @@ int hy_model_device_bytes(hy_model h, unsigned long long* bytes) {
   b += (double)m.esz * ((double)m.xsz * m.nxslots + (double)m.zsz * m.nzslots + (double)m.nt * m.nfslots);
   b += sizeof(double) * ((double)m.nt + std::max<double>((double)g.ny * g.nz * 4, 4096));  // kcomb, reductions
   if (m.fext) b += sizeof(double) * 3.0 * g.nreal();
+  if (m.vsc) b += sizeof(double) * 2.0 * g.nx;
+  if (m.rtab) b += sizeof(double) * 4.0 * (RT_N + 1);
   if (m.fdtab) b += sizeof(double) * (double)fd::NTAB * fd::NC;
   b += sizeof(double) * 6.0 * (m.probe_cap + m.aprobe_cap) + sizeof(PhysArgs) * XA_N + sizeof(Ctl);
   b += (double)g.scratch_bytes;

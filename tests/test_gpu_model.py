@@ -152,3 +152,31 @@ def test_mixed_steps_match_oracle(gpu, name):
     assert P.rel_l2(n - n0, om.n - n0) <= MIXED_TOL_DN, name
     for d in range(p.ny > 1 and (3 if p.nz > 1 else 2) or 1):
         assert P.rel_l2(mom[d], om.mom[d]) <= MIXED_TOL_DN, (name, d)
+
+
+def test_mixed_electrons_density_table_and_fallback(gpu):
+    """Mixed-precision electrons evaluate the Fermi-Dirac functional from a
+    per-model table over |n - n0| <= 0.2 n0 and fall back to the per-point
+    evaluation outside it.  A density with a 10 % rms excursion puts points
+    on both sides of that bound in every x-line kernel launch; the run must
+    still meet the mixed bars against the FP64 oracle."""
+    from hydft_amd.model import Model
+    kw, _ = CASES["elec3d"]
+    p = P.params(**kw)
+    st = P.structure(p.kind, p.gamma, p.kappa, p.rs)
+    gm = Model(p, structure=st, precision="mixed")
+    om = P.oracle_model(p, st)
+    n, mom, _ = P.smooth_fields(p, seed=11, amp_n=0.1, amp_u=0.01)
+    x = np.abs(n / P.N0 - 1.0)
+    assert (x > 0.2).mean() > 0.01 and (x < 0.19).mean() > 0.5  # both paths taken
+    gm.set_state(n, mom)
+    P.set_oracle_state(om, n, mom)
+    om.dt = p.dt
+    for _ in range(2):
+        om.advance()
+    gm.advance(2)
+    ng, momg, _ = gm.get_state()
+    assert P.rel_l2(ng, om.n) <= MIXED_TOL_N
+    assert P.rel_l2(ng - P.N0, om.n - P.N0) <= MIXED_TOL_DN
+    for d in range(3):
+        assert P.rel_l2(momg[d], om.mom[d]) <= MIXED_TOL_DN, d

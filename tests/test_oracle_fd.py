@@ -128,3 +128,39 @@ def test_field_kernel_forms_match_reference_forms():
     a_ref = fd_eval(7, ys)
     a_ker = fd_eval(12, ys)
     np.testing.assert_allclose(a_ker, a_ref, rtol=1e-13, atol=1e-13)
+
+
+def test_density_table_scheme():
+    """The mixed electron functional's density table: nodes x_i = -0.2 + i h
+    (h = 0.4/2048) of x = (n - n0)/n0 hold alpha - alpha0, xi and d xi/d n
+    from the reference evaluation (fermi_dirac.f90:225-267); 4-point Lagrange
+    interpolation at any |x| < 0.2 - 2h reproduces that evaluation to 1e-12
+    relative, across degenerate and classical alpha0."""
+    from hydft_amd.model import fd_eval
+    N, R = 2048, 0.2
+    h = 2 * R / N
+    rng = np.random.default_rng(7)
+
+    def factors(alpha, cn):
+        im, d1, d2 = fd_eval(0, alpha), fd_eval(3, alpha), fd_eval(4, alpha)
+        return d1 / im ** 2, (d2 * im - 2 * d1 * d1) / im ** 3 / im * (2.0 / cn)
+
+    for a0 in (-6.0, -0.5, 0.4, 3.0, 25.0):
+        cn = 0.37
+        n0 = cn * float(fd_eval(1, np.array([a0]))[0])
+        xn = -R + h * np.arange(N + 1)
+        an = fd_eval(7, n0 * (1 + xn) / cn)
+        tab = np.stack([an - a0, *factors(an, cn)])
+        xs = rng.uniform(-(R - 2 * h), R - 2 * h, 4000)
+        ar = fd_eval(7, n0 * (1 + xs) / cn)
+        ref = np.stack([ar - a0, *factors(ar, cn)])
+        u = (xs + R) / h
+        i = u.astype(int) - 1
+        t = u - (i + 1)
+        w = [-t * (t - 1) * (t - 2) / 6, (t + 1) * (t - 1) * (t - 2) / 2, -(t + 1) * t * (t - 2) / 2,
+             (t + 1) * t * (t - 1) / 6]
+        got = sum(w[k] * tab[:, i + k] for k in range(4))
+        scale = np.abs(ref).max(axis=1, keepdims=True)
+        assert np.abs(got - ref).max(axis=1).max() <= 1e-12 * scale.max(), a0
+        for c in (1, 2):
+            assert np.max(np.abs(got[c] - ref[c]) / np.abs(ref[c])) <= 1e-12, (a0, c)
