@@ -349,19 +349,34 @@ __device__ __attribute__((always_inline)) inline void pencil_inv_body(const Penc
     }
   } else {
     __shared__ cpx<T> lds[L * NA];
+    // the cross twiddles W_L^(n2 k1) from an LDS copy of the table: one
+    // global load per thread (issued with the column loads) instead of
+    // L1 - 1 dependent loads after the first DFT
+    // (issued after the column loads: the first DFT waits for those anyway)
+    __shared__ cpx<T> stw[L];
+    cpx<T> v[S::J1][L1];
 #pragma unroll
     for (int j = 0; j < S::J1; ++j) {
       const int n2 = g + TPC * j;
-      cpx<T> v[L1];
       int pos[L1];
 #pragma unroll
       for (int n1 = 0; n1 < L1; ++n1) pos[n1] = L2 * n1 + n2;
-      gather(std::integral_constant<int, L1>{}, pos, v);
-      rdft<T, L1, true>(v);
+      gather(std::integral_constant<int, L1>{}, pos, v[j]);
+    }
+#pragma unroll
+    for (int r = 0; r < (L + S::NT - 1) / S::NT; ++r) {
+      const int e = (int)threadIdx.x + S::NT * r;
+      if (e < L) stw[e] = gload_c<T>(twg, e);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < S::J1; ++j) {
+      const int n2 = g + TPC * j;
+      rdft<T, L1, true>(v[j]);
 #pragma unroll
       for (int k1 = 0; k1 < L1; ++k1) {
-        cpx<T> w = v[k1];
-        if (k1 > 0) w = cmulc(w, gload_c<T>(twg, (n2 * k1) % L));
+        cpx<T> w = v[j][k1];
+        if (k1 > 0) w = cmulc(w, stw[(n2 * k1) % L]);
         lds[(k1 * L2 + n2) * NA + al] = w;
       }
     }
@@ -426,19 +441,34 @@ __device__ __attribute__((always_inline)) inline void pencil_fwd_body(const Penc
     for (int p = 0; p < L; ++p) store(p, v[p]);
   } else {
     __shared__ cpx<T> lds[L * NA];
+    // the cross twiddles W_L^(n2 k1) from an LDS copy of the table: one
+    // global load per thread (issued with the column loads) instead of
+    // L1 - 1 dependent loads after the first DFT
+    // (issued after the column loads: the first DFT waits for those anyway)
+    __shared__ cpx<T> stw[L];
+    cpx<T> v[S::J1][L1];
 #pragma unroll
     for (int j = 0; j < S::J1; ++j) {
       const int n2 = g + TPC * j;
-      cpx<T> v[L1];
       int pos[L1];
 #pragma unroll
       for (int n1 = 0; n1 < L1; ++n1) pos[n1] = L2 * n1 + n2;
-      gather(std::integral_constant<int, L1>{}, pos, v);
-      rdft<T, L1, false>(v);
+      gather(std::integral_constant<int, L1>{}, pos, v[j]);
+    }
+#pragma unroll
+    for (int r = 0; r < (L + S::NT - 1) / S::NT; ++r) {
+      const int e = (int)threadIdx.x + S::NT * r;
+      if (e < L) stw[e] = gload_c<T>(twg, e);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < S::J1; ++j) {
+      const int n2 = g + TPC * j;
+      rdft<T, L1, false>(v[j]);
 #pragma unroll
       for (int k1 = 0; k1 < L1; ++k1) {
-        cpx<T> w = v[k1];
-        if (k1 > 0) w = cmul(w, gload_c<T>(twg, (n2 * k1) % L));
+        cpx<T> w = v[j][k1];
+        if (k1 > 0) w = cmul(w, stw[(n2 * k1) % L]);
         lds[(k1 * L2 + n2) * NA + al] = w;
       }
     }
