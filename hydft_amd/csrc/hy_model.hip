@@ -1041,7 +1041,7 @@ __global__ __launch_bounds__(256) void k_spec1_b(const K1Args* __restrict__ as) 
 struct K2Args {
   SpecArgs s;
   c64* S;                 // state: n, mom(3), Pi(6) (FP64)
-  c64* R;                 // R_old (same layout), or the output in rhs mode
+  void* R;                // R_old (same layout, transform type T), or the FP64 output in rhs mode
   const void* fl[6];      // F(flux_ij): 11 12 13 22 23 33
   const void* fh[3];      // F(H_i)
   const void* fu[3];      // F(u) (Newtonian stress)
@@ -1115,6 +1115,9 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
 #pragma unroll
       for (int c = 0; c < 6; ++c) r[4 + c] = ld<T>(a.fadv[c], e);
     }
+    // R_old is kept in the transform type: in mixed precision the RHS is
+    // formed from FP32 spectra (the continuity term excepted) and is
+    // FP32-accurate already; the update itself is FP64.
     auto put = [&](int f, c64 rf) {
       if (a.rhs_only) {
         gstc<double>(a.R, f * nt + e, rf);
@@ -1124,11 +1127,11 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
       if (a.stage == 0) {
         sv = sv + scal(rf, dt * ra);
       } else {
-        const c64 ro = gldc<double>(a.R, f * nt + e);
+        const c64 ro = cvt<double>(gldc<T>(a.R, f * nt + e));
         sv = sv + scal(scal(rf, ra) + scal(ro, rb), dt);
       }
       gstc<double>(a.S, f * nt + e, sv);
-      gstc<double>(a.R, f * nt + e, rf);
+      gstc<T>(a.R, f * nt + e, cvt<T>(rf));
     };
 #pragma unroll
     for (int f = 0; f < 4; ++f) put(f, r[f]);
@@ -1795,7 +1798,7 @@ static int launch_k2(Model& m, int stage_idx, int rhs_only) {
   std::memset(&a, 0, sizeof(a));
   a.s = spec_args(m);
   a.S = m.S;
-  a.R = rhs_only ? reinterpret_cast<c64*>(fslot(m, FS_RHS)) : m.R;
+  a.R = rhs_only ? fslot(m, FS_RHS) : (void*)m.R;
   const int fo[6] = {OA_F11, OA_F12, OA_F13, OA_F22, OA_F23, OA_F33};
   for (int i = 0; i < 6; ++i) a.fl[i] = fslot(m, FS_A(fo[i]));
   for (int i = 0; i < 3; ++i) a.fh[i] = fslot(m, FS_B(OB_H1 + i));
@@ -1813,7 +1816,9 @@ static int launch_k2(Model& m, int stage_idx, int rhs_only) {
     const int nfl = nd * (nd + 1) / 2;
     const int ns = m.evolve ? 10 : 4;
     double b = nt * es * (nfl + nd + (m.stress_newt ? nd : 0) + (m.evolve ? 6 : 0));
-    b += nt * 16.0 * (ns * 2 + (rhs_only ? ns : ns * 2));
+    // S read and written; R written (FP64 in rhs mode, else the transform
+    // type) and, after the first stage, read
+    b += nt * 16.0 * ns * 2 + nt * ns * (rhs_only ? 16.0 : es * (stage_idx > 0 ? 2.0 : 1.0));
     prof_begin(m, PC_K2, b);
   }
   const dim3 gr(nblocks(m.nt)), bl(256);
