@@ -173,6 +173,7 @@ struct PencilArgs {
   double dky; int ay_n;     // ky spacing; y stored count in the contiguous index (0 = none)
   int ay_h, ay_L;                // y stored-index mapping (h, L) when ay_n > 0
   int y_from_pencil;             // 1: ky comes from the pencil position (y pass)
+  int field_fastest;             // 1: workgroup order with the field index fastest (inverse passes)
   const int* halt;               // every workgroup returns when *halt != 0 (null: never)
 };
 
@@ -300,12 +301,25 @@ __device__ __attribute__((always_inline)) inline void pencil_inv_body(const Penc
   using S = PencilShape<T, L, NA_>;
   constexpr int L1 = S::L1, L2 = S::L2, TPC = S::TPC, NA = S::NA;
   const int al = threadIdx.x % NA, g = threadIdx.x / NA;
-  // 1-D grid of ntile * B * nf workgroups in XCD-aware order
+  // 1-D grid of ntile * B * nf workgroups in XCD-aware order.  With
+  // field_fastest (z inverse passes, whose fields often share an input: a
+  // field and its derivatives), the field index varies fastest, so fields
+  // of one input load the same tile back to back on one XCD, the later ones
+  // from its L2; otherwise the tile index does (neighbouring tiles share
+  // cache lines of a row).
   const int ntile = (a.A + NA - 1) / NA;
+  const unsigned nf = gridDim.x / ((unsigned)ntile * (unsigned)a.B);
   const unsigned w = xcd_work_index(blockIdx.x, gridDim.x);
-  const int tile = (int)(w % (unsigned)ntile);
-  const int b = (int)((w / (unsigned)ntile) % (unsigned)a.B);
-  const int fi = (int)(w / ((unsigned)ntile * (unsigned)a.B));
+  int fi, tile, b;
+  if (a.field_fastest) {
+    fi = (int)(w % nf);
+    tile = (int)((w / nf) % (unsigned)ntile);
+    b = (int)(w / (nf * (unsigned)ntile));
+  } else {
+    tile = (int)(w % (unsigned)ntile);
+    b = (int)((w / (unsigned)ntile) % (unsigned)a.B);
+    fi = (int)(w / ((unsigned)ntile * (unsigned)a.B));
+  }
   const int ga = tile * NA + al;
   const PencilField& f = a.f[fi];
   const bool live = ga < a.A;
@@ -794,6 +808,7 @@ int pencil_pass(const Grid& g, int axis, bool inverse, bool trunc, int A, int B,
   a.ax_x = trunc ? g.Kx : g.nkx;
   a.dky = twopi / g.ly;
   a.y_from_pencil = axis == 1 ? 1 : 0;
+  a.field_fastest = inverse && axis == 2 ? 1 : 0;
   if (axis == 2 && g.ny > 1) {
     a.ay_n = trunc ? g.Ky : g.ny;
     a.ay_h = trunc ? (g.Ky + 1) / 2 : g.ny;

@@ -1350,10 +1350,18 @@ static int FS_B(int o) { return 10 + o; }     // phase B outputs, 10..18
 static const int FS_MU = 19;
 static const int FS_RHS = 20;                 // rhs-only output: 10 slots of c64 (20..)
 
+// Compulsory bytes of a pencil pass: every distinct input read once (fields
+// that share an input, a field and its derivatives, count it once), every
+// output written once.
 static double pencil_bytes(const Model& m, const std::vector<PencilField>& f, size_t A, size_t B, size_t L,
                            size_t K) {
   double b = 0.0;
-  for (auto& x : f) b += (double)A * B * (K * (x.in_f64 ? 16.0 : (double)m.esz) + L * (double)m.esz);
+  for (size_t i = 0; i < f.size(); ++i) {
+    bool seen = false;
+    for (size_t j = 0; j < i && !seen; ++j) seen = f[j].in == f[i].in;
+    if (!seen) b += (double)A * B * K * (f[i].in_f64 ? 16.0 : (double)m.esz);
+    b += (double)A * B * L * (double)m.esz;
+  }
   return b;
 }
 
