@@ -102,16 +102,29 @@ def pencil_plans():
     return [list(buf[8 * i:8 * i + 8]) for i in range(n)]
 
 
-def copy_probe(mib):
-    """A plain copy kernel of known size: reads and writes mib MiB once per launch, 3 launches."""
+def copy_probe(mib, timed=False):
+    """A plain copy kernel of known size: reads and writes mib MiB once per launch, 3 launches.
+    timed: also time 10 more launches with events and report the copy rate, the
+    practical HBM ceiling of a kernel that reads and writes once (read + write
+    bytes over time)."""
     import torch
     x = torch.ones(mib * 262144, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
     for _ in range(3):
         y.copy_(x)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_read": mib * 1048576, "copy_probe_bytes_written": mib * 1048576,
-                      "launches": 3}))
+    out = {"copy_probe_bytes_read": mib * 1048576, "copy_probe_bytes_written": mib * 1048576, "launches": 3}
+    if timed:
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            y.copy_(x)
+        e1.record()
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) * 1e-3 / 10
+        out["copy_GBps"] = 2.0 * mib * 1048576 / sec / 1e9
+        out["launches"] += 10
+    print(json.dumps(out))
     return 0
 
 
@@ -463,7 +476,7 @@ def main():
         return 1
 
     if args.copy_probe:
-        return copy_probe(args.copy_probe)
+        return copy_probe(args.copy_probe, timed=True)
     traffic, traffic_note = (None, "skipped") if args.no_traffic else pmc_traffic(args)
     import torch
     from hydft_amd import abi

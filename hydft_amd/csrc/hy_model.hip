@@ -1066,10 +1066,15 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nt; e += (size_t)gridDim.x * blockDim.x) {
     double k[3];
     spec_k(a.s, e, k);
+    // the state of this mode, loaded once (the update below stores S[0]
+    // before it reads S[1..3]; reloading them after that store re-reads HBM)
+    c64 sv0[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) sv0[f] = gldc<double>(a.S, f * nt + e);
     // continuity: dn = -div(mom)/m (model.f90:95-96)
     c64 km = mk<double>(0, 0);
 #pragma unroll
-    for (int d = 0; d < ND; ++d) km = km + scal(gldc<double>(a.S, (1 + d) * nt + e), k[d]);
+    for (int d = 0; d < ND; ++d) km = km + scal(sv0[1 + d], k[d]);
     c64 r[10];
     r[0] = mk<double>(km.y * a.inv_mass, -km.x * a.inv_mass);
     // stress divergence f_i = d_j Pi_ij
@@ -1123,7 +1128,7 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
         gstc<double>(a.R, f * nt + e, rf);
         return;
       }
-      c64 sv = gldc<double>(a.S, f * nt + e);
+      c64 sv = f < 4 ? sv0[f] : gldc<double>(a.S, f * nt + e);
       if (a.stage == 0) {
         sv = sv + scal(rf, dt * ra);
       } else {
