@@ -124,6 +124,12 @@ def copy_probe(mib, timed=False):
         sec = e0.elapsed_time(e1) * 1e-3 / 10
         out["copy_GBps"] = 2.0 * mib * 1048576 / sec / 1e9
         out["launches"] += 10
+        e0.record()
+        for _ in range(10):
+            y.fill_(1.0)
+        e1.record()
+        torch.cuda.synchronize()
+        out["fill_GBps"] = mib * 1048576 / (e0.elapsed_time(e1) * 1e-3 / 10) / 1e9
     print(json.dumps(out))
     return 0
 
