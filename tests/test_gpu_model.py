@@ -54,6 +54,15 @@ CASES = {
                               eta=0.3, xi=0.1, dt=0.01), 2),
     "elec3d_nx256": (dict(kind="electron", gamma=1.0, rs=1.86, nx=256, ny=8, nz=4, lx=60.0, ly=8.0, lz=6.0,
                           eta=0.1, dt=0.01), 2),
+    # driven ions on 3-D grids with 32-point x lines: the fixed x-line forms
+    # (ion phase A; phase B with the tabulated wave force, Newtonian and Maxwell)
+    "ion3d_driven_nx32": (dict(nx=32, ny=8, nz=8, lx=20.0, ly=10.0, lz=10.0, gamma=2.0, kappa=0.1, eta=0.3,
+                               vext="driven", vext_amplitude=1e-3, vext_mode=(1, 1, 1), vext_omega=0.4,
+                               vext_ramp=0.05, dt=0.02), 3),
+    "ion3d_maxwell_driven_nx64": (dict(nx=64, ny=8, nz=4, lx=20.0, ly=10.0, lz=5.0, gamma=20.0, kappa=1.0,
+                                       eta=0.2, closure="maxwell", tau_model="constant", tau=0.5,
+                                       vext="driven", vext_amplitude=1e-3, vext_mode=(2, 1, 1), vext_omega=0.7,
+                                       vext_ramp=0.05, dt=0.005), 3),
 }
 
 # FP32 transforms with the FP64 RK3 state: density error against the FP64
