@@ -458,6 +458,10 @@ def main():
     ap.add_argument("--scan", type=int, default=512, metavar="NRUNS",
                     help="batched driven-mode scan: number of runs (0: skip)")
     ap.add_argument("--scan-steps", type=int, default=2000)
+    ap.add_argument("--stream-rate", type=float, default=50.0, metavar="PER_S",
+                    help="Poisson arrival rate of the serving stream (requests/s)")
+    ap.add_argument("--stream-only", type=int, default=0, metavar="NREQ",
+                    help="run only the serving stream of NREQ requests at --stream-rate")
     ap.add_argument("--stream", type=int, default=100, metavar="NREQ",
                     help="serving stream: number of requests (0: skip)")
     ap.add_argument("--serve-steps", type=int, default=400, help="steps per run of --batch-only")
@@ -476,6 +480,11 @@ def main():
         return hnc_only(args.hnc_only)
     if args.batch_only:
         return batch_only(args.batch_only, args.serve_steps)
+    if args.stream_only:
+        from hydft_amd import abi
+        abi.call("hy_init", 0)
+        print(json.dumps(stream_sample(args.stream_only, rate=args.stream_rate)))
+        return 0
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; runs on more than one GPU are not part of it "
               "(independent runs would shard with no exchange).", file=sys.stderr)
@@ -552,7 +561,7 @@ def main():
     if args.scan > 0:
         extra["scan"] = scan_sample(args.scan, args.scan_steps)
     if args.stream > 0:
-        extra["stream"] = stream_sample(args.stream)
+        extra["stream"] = stream_sample(args.stream, rate=args.stream_rate)
     if not args.no_maxwell:
         extra["maxwell_128"] = maxwell_sample()
     cpu = None
