@@ -95,7 +95,6 @@ const double* fd::host_table() {
 struct Ctl {
   double t, dt, ts;          // model time, step size, current stage time
   double relax_fac, relax_phi;
-  double nf[3];              // mean(n f_ext,i) of the current stage
   double umax;
   double t_vext;             // time of the last v_ext evaluation (stage start)
   int step;
@@ -915,35 +914,6 @@ __global__ void k_dt(DtK k) { dt_body(k); }
 __global__ void k_dt_b(const DtK* __restrict__ ks) { dt_body(ks[blockIdx.z]); }
 
 // sum over lines of `ncomp` partials, scaled: out[c] = scale * sum_l red[l*ncomp + c]
-struct RedK {
-  const double* red;
-  int nlines, ncomp;
-  double scale;
-  double* out;
-  const int* halt;
-};
-__device__ __attribute__((always_inline)) inline void reduce_lines_body(const RedK& k) {
-  const double* __restrict__ red = k.red;
-  const int nlines = k.nlines, ncomp = k.ncomp;
-  const double scale = k.scale;
-  double* __restrict__ out = k.out;
-  if (gate_closed(k.halt)) return;  // uniform over the block
-  __shared__ double sh[256];
-  for (int c = 0; c < ncomp; ++c) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nlines; i += blockDim.x) s += red[(size_t)i * ncomp + c];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[c] = sh[0] * scale;
-    __syncthreads();
-  }
-}
-__global__ void k_reduce_lines(RedK k) { reduce_lines_body(k); }
-__global__ void k_reduce_lines_b(const RedK* __restrict__ ks) { reduce_lines_body(ks[blockIdx.z]); }
 
 // ------------------------------------------------------------ spectral kernels
 struct SpecArgs {
@@ -1048,6 +1018,9 @@ struct K2Args {
   const void* fadv[6];    // F(adv_c) (Maxwell)
   double mass, inv_mass, eta, xi;
   int stress_newt, evolve, ext_active;
+  const double* red;      // external force: per-line partial sums of n f_i (3 per line, phase B)
+  int nlines;             // their line count
+  double red_scale;       // 1 / (number of grid points)
   int stage;              // 0..2
   int rhs_only;
   const Ctl* ctl;
@@ -1063,6 +1036,25 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
   constexpr int idx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};  // stress.f90:92
   const double dt = a.ctl->dt;
   const double ra = c_rk_a[a.stage], rb = c_rk_b[a.stage];
+  // mean(n f_ext) of this stage (the k = 0 momentum RHS, model.f90:119-122):
+  // the first workgroup sums phase B's per-line partial sums itself before it
+  // handles mode 0, which only its first thread does
+  __shared__ double nf[3];
+  if (a.ext_active && blockIdx.x == 0) {
+    __shared__ double sh[256];
+    for (int c = 0; c < 3; ++c) {
+      double v = 0.0;
+      for (int i = threadIdx.x; i < a.nlines; i += blockDim.x) v += a.red[(size_t)i * 3 + c];
+      sh[threadIdx.x] = v;
+      __syncthreads();
+      for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) nf[c] = sh[0] * a.red_scale;
+      __syncthreads();
+    }
+  }
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nt; e += (size_t)gridDim.x * blockDim.x) {
     double k[3];
     spec_k(a.s, e, k);
@@ -1112,7 +1104,7 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
         }
         // k = 0: only the external force survives (the internal force has its mean
         // removed, model.f90:119-122)
-        if (e == 0) acc = mk<double>(a.ext_active ? a.ctl->nf[i] : 0.0, 0.0);
+        if (e == 0) acc = mk<double>(a.ext_active ? nf[i] : 0.0, 0.0);
       }
       r[1 + i] = acc;
     }
@@ -1767,10 +1759,6 @@ static int stage(Model& m, int s, int rhs_only) {
   if (m.evolve) for (int c = 0; c < 6; ++c) addB(OB_ADV1 + c);
   HY_TRY(run_xk(m, XK_B, XB_STAGE, b));
   HY_TRY(fwd_chain(m, fwdb));
-  if (m.vext_kind != HY_VEXT_NONE) {
-    const RedK rk{m.red, g.ny * g.nz, 3, 1.0 / (double)g.nreal(), m.ctl->nf, &m.ctl->halt};
-    HY_TRY(launch(HY_FN(k_reduce_lines), HY_FN(k_reduce_lines_b), dim3(1), dim3(256), m.st, rk));
-  }
   // ---- K2: RHS assembly and the RK update
   HY_TRY(launch_k2(m, s, rhs_only));
   return HY_OK;
@@ -1820,6 +1808,9 @@ static int launch_k2(Model& m, int stage_idx, int rhs_only) {
   a.mass = m.mass; a.inv_mass = 1.0 / m.mass; a.eta = m.eta; a.xi = m.xi;
   a.stress_newt = m.stress_newt; a.evolve = m.evolve;
   a.ext_active = m.vext_kind != HY_VEXT_NONE;
+  a.red = m.red;
+  a.nlines = m.g.ny * m.g.nz;
+  a.red_scale = 1.0 / (double)m.g.nreal();
   a.stage = stage_idx;
   a.rhs_only = rhs_only;
   a.ctl = m.ctl;
@@ -1832,6 +1823,7 @@ static int launch_k2(Model& m, int stage_idx, int rhs_only) {
     // S read and written; R written (FP64 in rhs mode, else the transform
     // type) and, after the first stage, read
     b += nt * 16.0 * ns * 2 + nt * ns * (rhs_only ? 16.0 : es * (stage_idx > 0 ? 2.0 : 1.0));
+    if (m.vext_kind != HY_VEXT_NONE) b += 8.0 * 3.0 * m.g.ny * m.g.nz;  // phase B's per-line force sums
     prof_begin(m, PC_K2, b);
   }
   const dim3 gr(nblocks(m.nt)), bl(256);
