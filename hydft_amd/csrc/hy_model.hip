@@ -112,18 +112,35 @@ __constant__ double c_rk_b[3] = {0.0, -17.0 / 60.0, -25.0 / 60.0};
 static const double h_rk_a[3] = {32.0 / 60.0, 25.0 / 60.0, 45.0 / 60.0};
 static const double h_rk_b[3] = {0.0, -17.0 / 60.0, -25.0 / 60.0};
 
-enum CtlOp { CTL_STAGE_END = 0, CTL_STEP_END = 1, CTL_STEP_BEGIN = 2, CTL_SEQ_BEGIN = 3, CTL_OPEN = 4 };
+enum CtlOp { CTL_STEP_BEGIN = 2, CTL_SEQ_BEGIN = 3, CTL_OPEN = 4 };  // the ends of stages and steps: K2
 struct CtlK {
   Ctl* c;
-  int op, s;
-  double tau;
-  int evolve;
+  int op;
   double tend;
 };
+// The ends of a stage and of a step (time, stage time, Maxwell relaxation
+// factors of the next stage, step count).  K2 applies them from its first
+// thread: nothing in K2 reads the fields they change, and the next stage's
+// kernels follow K2 on the stream.
+__device__ inline void ctl_stage_end(Ctl* c, int s, double tau, int evolve) {
+  c->t_vext = c->ts;
+  double dts = (c_rk_a[s] + c_rk_b[s]) * c->dt;
+  c->ts = c->ts + dts;
+  if (evolve) {
+    double fac = exp(-dts / tau);
+    c->relax_fac = fac;
+    c->relax_phi = (tau / dts) * (1.0 - fac);
+  }
+}
+__device__ inline void ctl_step_end(Ctl* c) {
+  c->t = c->t + c->dt;
+  c->step += 1;
+  c->ts = c->t;
+}
 __device__ __attribute__((always_inline)) inline void ctl_body(const CtlK& k) {
   Ctl* c = k.c;
-  const int op = k.op, s = k.s, evolve = k.evolve;
-  const double tau = k.tau, tend = k.tend;
+  const int op = k.op;
+  const double tend = k.tend;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (op == CTL_SEQ_BEGIN) {
     c->halt = (c->stop > 0 && c->step >= c->stop) ? 1 : 0;
@@ -136,19 +153,6 @@ __device__ __attribute__((always_inline)) inline void ctl_body(const CtlK& k) {
   if (c->halt) return;
   if (op == CTL_STEP_BEGIN) {
     if (tend > 0.0 && c->t + c->dt > tend) c->dt = tend - c->t;  // src/hydft.f90:44
-    c->ts = c->t;
-  } else if (op == CTL_STAGE_END) {
-    c->t_vext = c->ts;
-    double dts = (c_rk_a[s] + c_rk_b[s]) * c->dt;
-    c->ts = c->ts + dts;
-    if (evolve) {
-      double fac = exp(-dts / tau);
-      c->relax_fac = fac;
-      c->relax_phi = (tau / dts) * (1.0 - fac);
-    }
-  } else if (op == CTL_STEP_END) {
-    c->t = c->t + c->dt;
-    c->step += 1;
     c->ts = c->t;
   }
 }
@@ -1023,7 +1027,9 @@ struct K2Args {
   double red_scale;       // 1 / (number of grid points)
   int stage;              // 0..2
   int rhs_only;
-  const Ctl* ctl;
+  int end_ops;            // 1: end the stage; 2: also end the step (first thread of the first workgroup)
+  double tau;             // Maxwell relaxation time (the stage end's factors)
+  Ctl* ctl;
 };
 
 // ND is a template parameter so that every loop below unrolls and every
@@ -1036,6 +1042,10 @@ __device__ __attribute__((always_inline)) inline void spec2_body(const K2Args& a
   constexpr int idx[3][3] = {{0, 3, 4}, {3, 1, 5}, {4, 5, 2}};  // stress.f90:92
   const double dt = a.ctl->dt;
   const double ra = c_rk_a[a.stage], rb = c_rk_b[a.stage];
+  if (a.end_ops && blockIdx.x == 0 && threadIdx.x == 0 && !a.ctl->halt) {
+    ctl_stage_end(a.ctl, a.stage, a.tau, a.evolve);
+    if (a.end_ops == 2) ctl_step_end(a.ctl);
+  }
   // mean(n f_ext) of this stage (the k = 0 momentum RHS, model.f90:119-122):
   // the first workgroup sums phase B's per-line partial sums itself before it
   // handles mode 0, which only its first thread does
@@ -1693,10 +1703,11 @@ static int stage_inputs(Model& m, PhysArgs& a, bool with_grad) {
 }
 
 static int launch_k1(Model& m, int relax);
-static int launch_k2(Model& m, int stage, int rhs_only);
+static int launch_k2(Model& m, int stage, int rhs_only, int end_ops);
 
-// One RHS evaluation up to K2; the stage time is ctl->ts.
-static int stage(Model& m, int s, int rhs_only) {
+// One RHS evaluation up to K2; the stage time is ctl->ts.  end_ops: the
+// control updates K2 applies after the stage (see K2Args).
+static int stage(Model& m, int s, int rhs_only, int end_ops = 0) {
   const Grid& g = m.g;
   const int nd = g.ndim;
   // ---- phase A
@@ -1760,7 +1771,7 @@ static int stage(Model& m, int s, int rhs_only) {
   HY_TRY(run_xk(m, XK_B, XB_STAGE, b));
   HY_TRY(fwd_chain(m, fwdb));
   // ---- K2: RHS assembly and the RK update
-  HY_TRY(launch_k2(m, s, rhs_only));
+  HY_TRY(launch_k2(m, s, rhs_only, end_ops));
   return HY_OK;
 }
 
@@ -1794,7 +1805,7 @@ static int launch_k1(Model& m, int relax) {
   return HY_OK;
 }
 
-static int launch_k2(Model& m, int stage_idx, int rhs_only) {
+static int launch_k2(Model& m, int stage_idx, int rhs_only, int end_ops) {
   K2Args a;
   std::memset(&a, 0, sizeof(a));
   a.s = spec_args(m);
@@ -1813,6 +1824,8 @@ static int launch_k2(Model& m, int stage_idx, int rhs_only) {
   a.red_scale = 1.0 / (double)m.g.nreal();
   a.stage = stage_idx;
   a.rhs_only = rhs_only;
+  a.end_ops = rhs_only ? 0 : end_ops;
+  a.tau = m.tau;
   a.ctl = m.ctl;
   {
     const double nt = (double)m.nt, es = (double)m.esz;
@@ -1894,19 +1907,20 @@ static int compute_dt_dev(Model& m) {
   return launch(HY_FN(k_dt), HY_FN(k_dt_b), dim3(1), dim3(256), m.st, dk);
 }
 
-static int ctl_op(Model& m, int op, int s, double tend) {
-  const CtlK k{m.ctl, op, s, m.tau, m.evolve, tend};
+static int ctl_op(Model& m, int op, double tend) {
+  const CtlK k{m.ctl, op, tend};
   return launch(HY_FN(k_ctl), HY_FN(k_ctl_b), dim3(1), dim3(1), m.st, k);
 }
 
+// One RK3 step: the step's start on the control block, then three stages,
+// whose K2 launches end each stage and, after the third, the step.
 static int step_once(Model& m, double tend) {
-  HY_TRY(ctl_op(m, CTL_STEP_BEGIN, 0, tend));
+  HY_TRY(ctl_op(m, CTL_STEP_BEGIN, tend));
   for (int s = 0; s < 3; ++s) {
-    HY_TRY(stage(m, s, 0));
-    HY_TRY(ctl_op(m, CTL_STAGE_END, s, tend));
+    HY_TRY(stage(m, s, 0, s < 2 ? 1 : 2));
     if (m.evolve) m.relax_pending = 1;
   }
-  return ctl_op(m, CTL_STEP_END, 0, tend);
+  return HY_OK;
 }
 
 static int read_ctl(Model& m, Ctl& c) {
@@ -2590,7 +2604,7 @@ int hy_model_set_dt(hy_model h, double dt) {
 int hy_model_compute_dt(hy_model h, double* h_dt) {
   if (!h) return set_error(HY_ERR_INVALID, "null argument");
   Model& m = h->m;
-  HY_TRY(ctl_op(m, CTL_OPEN, 0, 0.0));
+  HY_TRY(ctl_op(m, CTL_OPEN, 0.0));
   HY_TRY(finish_relax(m));
   HY_TRY(compute_dt_dev(m));
   Ctl c;
@@ -2604,7 +2618,7 @@ int hy_model_advance(hy_model h, int nsteps, int dt_every, double tend, const in
                      double* h_probe, int* h_taken) {
   if (!h || nsteps < 0) return set_error(HY_ERR_INVALID, "hy_model_advance: bad argument");
   Model& m = h->m;
-  HY_TRY(ctl_op(m, CTL_OPEN, 0, 0.0));
+  HY_TRY(ctl_op(m, CTL_OPEN, 0.0));
   const Grid& g = m.g;
   long long pidx = -1;
   if (probe3) {
@@ -2713,7 +2727,7 @@ static int prepare_steps(Model& m, int nsteps, int dt_every, const int* probe3, 
 // takes them, each step followed by its probe row into the ring; ends with
 // no pending relaxation and at the start of the dt cadence again.
 static int emit_steps(Model& m, int nsteps, int dt_every, long long pidx, bool probe, int ring) {
-  HY_TRY(ctl_op(m, CTL_SEQ_BEGIN, 0, 0.0));
+  HY_TRY(ctl_op(m, CTL_SEQ_BEGIN, 0.0));
   for (int i = 0; i < nsteps; ++i) {
     if (dt_every > 0) {
       if (m.irecompute == 0) {
@@ -3147,7 +3161,7 @@ int hy_model_probe_ring(hy_model h, int ring, double* h_probe) {
 int hy_model_rhs(hy_model h, double t, double* h_dn, double* h_dmom3, double* h_dpi6) {
   if (!h) return set_error(HY_ERR_INVALID, "null argument");
   Model& m = h->m;
-  HY_TRY(ctl_op(m, CTL_OPEN, 0, 0.0));
+  HY_TRY(ctl_op(m, CTL_OPEN, 0.0));
   HY_TRY(finish_relax(m));
   Ctl c;
   HY_TRY(read_ctl(m, c));
