@@ -552,15 +552,16 @@ struct PhysArgs {
 // every choice from the argument block.  The other forms are for 3-D grids,
 // whose x-line inputs are all compute-type slots with no k = 0 offset and a
 // multiplier fixed by their role (XMul): XF_ANY3 fixes that layout (phase B
-// of any model), XF_NOF3 also the absence of an external force (phase B
-// without the force code: 102 against 73 VGPRs at 256 points in FP32);
-// XF_IDEAL also fixes phase A of a classical ion model (the
-// ideal term as the only pointwise functional; mu, u and the six fluxes
-// emitted in the order the host lists them).  Fixing these at compile time
+// of any model; phase A of the other functionals, FP32), XF_NOF3 also the
+// absence of an external force (phase B without the force code: 102 against
+// 73 VGPRs at 256 points in FP32); XF_IDEAL also fixes phase A of a classical
+// ion model (the ideal term as the only pointwise functional; mu, u and the
+// six fluxes emitted in the order the host lists them).  Fixing these at compile time
 // removes the uniform branches on the inputs' types and on the output codes,
 // with the register moves at their joins.  (A fixed form of the electron
-// functional was tried: its straight-line code needed more than the 128
-// VGPRs of 4 waves per SIMD and spilled.)
+// functional and output list was tried: its straight-line code needed more
+// than the 128 VGPRs of 4 waves per SIMD and spilled; the electrons take
+// XF_ANY3, which keeps both run-time.)
 enum XFun { XF_RUNTIME = 0, XF_IDEAL = 1, XF_ANY3 = 2, XF_NOF3 = 3 };
 template <int F>
 struct XCfg {
@@ -586,8 +587,8 @@ struct XCfg {
 template <class T, int NX, class X, class C>
 __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs* __restrict__ pa, const cpx<T>* __restrict__ twg) {
   constexpr bool FIX = C::IDEAL;
-  HY_XLINE_PREAMBLE(FIX)
-  constexpr int MN = FIX ? XM_NONE : -1, MK = FIX ? XM_IK : -1;
+  HY_XLINE_PREAMBLE(C::D3)
+  constexpr int MN = C::MN, MK = C::MK;
   const bool ideal = FIX || a.ideal != 0;
   const bool tf = !FIX && a.tf != 0;
   const bool grad = !FIX && a.grad != 0;
@@ -1493,6 +1494,11 @@ static int launch_xk(int which, bool maxwell, int cfg, const PhysArgs* a, int nl
       if (cfg == XF_IDEAL)
         return launch_x(HY_FN((k_phaseA<T, NX, X, XCfg<XF_IDEAL>>)), HY_FN((k_phaseA_b<T, NX, X, XCfg<XF_IDEAL>>)),
                         grid, s, a, tw);
+      if constexpr (sizeof(T) == 4) {
+        if (cfg == XF_ANY3)
+          return launch_x(HY_FN((k_phaseA<T, NX, X, XCfg<XF_ANY3>>)), HY_FN((k_phaseA_b<T, NX, X, XCfg<XF_ANY3>>)),
+                          grid, s, a, tw);
+      }
     }
     return launch_x(HY_FN((k_phaseA<T, NX, X, CR>)), HY_FN((k_phaseA_b<T, NX, X, CR>)), grid, s, a, tw);
   }
@@ -1552,10 +1558,12 @@ static int xk_cfg(const Model& m, int which, const PhysArgs& a) {
   in(a.n, XM_NONE);
   if (which == XK_A) {
     for (int d = 0; d < 3; ++d) in(a.m[d], XM_NONE);
-    if (!(a.ideal && !a.tf && !a.grad) || a.nout != 10) return XF_RUNTIME;
+    if (a.grad) { in(a.gx, XM_IK); in(a.gy, XM_NONE); in(a.gz, XM_NONE); in(a.lap, XM_NONE); }
+    if (!ok) return XF_RUNTIME;
+    if (!(a.ideal && !a.tf && !a.grad) || a.nout != 10) return XF_ANY3;
     static const int order[10] = {OA_MU, OA_U1, OA_U2, OA_U3, OA_F11, OA_F12, OA_F13, OA_F22, OA_F23, OA_F33};
     for (int o = 0; o < 10; ++o) ok = ok && a.code[o] == order[o];
-    return ok ? XF_IDEAL : XF_RUNTIME;
+    return ok ? XF_IDEAL : XF_ANY3;
   }
   in(a.mux, XM_IK); in(a.muy, XM_NONE); in(a.muz, XM_NONE);
   if (a.evolve) {
