@@ -971,7 +971,7 @@ struct K1Args {
   int evolve, relax;
   c64* P;                 // state Pi_hat (6)
   const void* fu[3];      // F(u) of this stage (compute type)
-  c64* uold;              // u_hat at the previous stage start (3, FP64)
+  void* uold;             // u_hat at the previous stage start (3, transform type: F(u) is stored in it)
   double eta, xi;
   const Ctl* ctl;
 };
@@ -991,7 +991,8 @@ __device__ __attribute__((always_inline)) inline void spec1_body(const K1Args& a
       c64 un[3];
       for (int d = 0; d < 3; ++d) un[d] = d < a.s.nd ? ld<T>(a.fu[d], e) : mk<double>(0, 0);
       if (a.relax) {  // model.f90:209-214, applied with this stage's u = u after the update
-        c64 uo[3] = {gldc<double>(a.uold, e), gldc<double>(a.uold, nt + e), gldc<double>(a.uold, 2 * nt + e)};
+        c64 uo[3] = {cvt<double>(gldc<T>(a.uold, e)), cvt<double>(gldc<T>(a.uold, nt + e)),
+                     cvt<double>(gldc<T>(a.uold, 2 * nt + e))};
         c64 p0n[6], p0o[6];
         pi0_hat(un, k, a.eta, a.xi, p0n);
         pi0_hat(uo, k, a.eta, a.xi, p0o);
@@ -1001,7 +1002,7 @@ __device__ __attribute__((always_inline)) inline void spec1_body(const K1Args& a
           gstc<double>(a.P, c * nt + e, p0n[c] - scal(p0n[c] - p0o[c], phi) + scal(pc - p0o[c], fac));
         }
       }
-      for (int d = 0; d < 3; ++d) gstc<double>(a.uold, d * nt + e, un[d]);
+      for (int d = 0; d < 3; ++d) gstc<T>(a.uold, d * nt + e, cvt<T>(un[d]));  // exact: un is F(u) in T
     }
   }
 }
@@ -1802,7 +1803,9 @@ static int launch_k1(Model& m, int relax) {
   {
     const double nt = (double)m.nt, es = (double)m.esz;
     double b = nt * (es + 16.0 + 8.0 + es);
-    if (m.evolve) b += nt * (3 * es + 3 * 16.0 * 2 + (relax ? 6 * 16.0 * 2 : 0.0));
+    // Maxwell: F(u) read and stored as the stage-start u; with a relaxation,
+    // the previous stage-start u read and Pi read and written
+    if (m.evolve) b += nt * (3 * es + 3 * es + (relax ? 3 * es + 6 * 16.0 * 2 : 0.0));
     prof_begin(m, PC_K1, b);
   }
   if (mixed(m)) HY_TRY(launch(HY_FN(k_spec1<float>), HY_FN(k_spec1_b<float>), dim3(nblocks(m.nt)), dim3(256), m.st, a));
