@@ -257,13 +257,16 @@ __device__ inline void gather_any(const void* p, size_t nel, const unsigned* idx
 // (pencil_plan below) times the candidates of each pass shape once and keeps
 // the fastest.  The default (the choice without timing) takes 64 columns per
 // block for one-thread columns; FP32 takes 16 for columns of at most 48 points,
-// 24 at 64 and 128 points and 32 otherwise; FP64 takes 8.
+// 24 at 64 and 128 points and 32 otherwise; FP64 takes 8, or as many as fill
+// one wave where a column has few threads (32 at 24 points, 16 at 32, 48 and
+// 96 points).
 template <class T, int L>
 __host__ __device__ constexpr int pencil_tpc() { return col_plan(L, (int)sizeof(cpx<T>)).TPC; }
 template <class T, int L>
 __host__ __device__ constexpr int pencil_na_default() {
   return pencil_tpc<T, L>() == 1 ? 64
-         : (sizeof(T) == 4 ? (L <= 48 ? 16 : (L == 64 || L == 128 ? 24 : 32)) : 8);
+         : (sizeof(T) == 4 ? (L <= 48 ? 16 : (L == 64 || L == 128 ? 24 : 32))
+                           : (64 / pencil_tpc<T, L>() > 8 ? 64 / pencil_tpc<T, L>() : 8));
 }
 // a candidate: at most 1024 threads and 64 KiB of LDS per workgroup
 template <class T, int L, int NA>
