@@ -112,6 +112,7 @@ _SIGS = {
     "hy_copy_to_device": [_P, _P, C.c_ulonglong],
     "hy_copy_to_host": [_P, _P, C.c_ulonglong],
     "hy_model_profile": [_P, _I],
+    "hy_model_profile_classes": [_P, C.c_uint],
     "hy_model_profile_read": [_P, _I, _P, _P, _P],
     "hy_model_graph_build": [_P, _I, _I, _PI, _I],
     "hy_model_graph_launch": [_P, _I],

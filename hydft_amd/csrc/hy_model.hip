@@ -1294,8 +1294,11 @@ struct Model {
   // host mirror of the time
   double h_t = 0, h_dt = 0;
   int h_step = 0;
-  // per-kernel-class profiling (HIP events on the model stream)
-  bool prof = false;
+  // per-kernel-class profiling (HIP events on the model stream): bit c of
+  // prof set records the launches of class c; prof_open marks a begin event
+  // whose end is still to come
+  unsigned prof = 0;
+  bool prof_open = false;
   struct Ev { int cls; hipEvent_t a, b; double bytes; };
   std::vector<Ev> evs;
   std::vector<hipEvent_t> pool;
@@ -1312,14 +1315,16 @@ static hipEvent_t prof_event(Model& m) {
   return e;
 }
 static void prof_begin(Model& m, int cls, double bytes) {
-  if (!m.prof) return;
+  if (!((m.prof >> cls) & 1u)) return;
   Model::Ev ev{cls, prof_event(m), prof_event(m), bytes};
   (void)hipEventRecord(ev.a, m.st);
   m.evs.push_back(ev);
+  m.prof_open = true;
 }
 static void prof_end(Model& m) {
-  if (!m.prof || m.evs.empty()) return;
+  if (!m.prof_open) return;
   (void)hipEventRecord(m.evs.back().b, m.st);
+  m.prof_open = false;
 }
 
 static bool mixed(const Model& m) { return m.prec == HY_PREC_MIXED; }
@@ -2763,8 +2768,8 @@ int hy_model_graph_build(hy_model h, int nsteps, int dt_every, const int* probe3
   HY_TRY(prepare_steps(m, nsteps, dt_every, probe3, ring));
   if (m.gexec) { (void)hipGraphExecDestroy(m.gexec); m.gexec = nullptr; }
   if (m.graph) { (void)hipGraphDestroy(m.graph); m.graph = nullptr; }
-  const bool prof = m.prof;
-  m.prof = false;
+  const unsigned prof = m.prof;
+  m.prof = 0;
   HY_HIP(hipStreamBeginCapture(m.st, hipStreamCaptureModeThreadLocal));
   const int rc = emit_steps(m, nsteps, dt_every, pidx, probe3 != nullptr, ring);
   hipGraph_t graph = nullptr;
@@ -2894,8 +2899,8 @@ int hy_batch_create(const hy_model* ms, int n, int nsteps, int dt_every, const i
     long long pidx = -1;
     HY_TRY(probe_index(m, probe3, &pidx));
     HY_TRY(prepare_steps(m, nsteps, dt_every, probe3, ring));
-    const bool prof = m.prof;
-    m.prof = false;
+    const unsigned prof = m.prof;
+    m.prof = 0;
     recorder() = &recs[i];
     const int rc = emit_steps(m, nsteps, dt_every, pidx, probe3 != nullptr, ring);
     recorder() = nullptr;
@@ -3303,15 +3308,18 @@ int hy_model_budgets(hy_model h, double* out6) {
   return HY_OK;
 }
 
-int hy_model_profile(hy_model h, int enable) {
+int hy_model_profile_classes(hy_model h, unsigned mask) {
   if (!h) return set_error(HY_ERR_INVALID, "null argument");
   Model& m = h->m;
   HY_HIP(hipStreamSynchronize(m.st));
   for (auto& e : m.evs) { m.pool.push_back(e.a); m.pool.push_back(e.b); }
   m.evs.clear();
-  m.prof = enable != 0;
+  m.prof = mask & ((1u << PC_N) - 1u);
+  m.prof_open = false;
   return HY_OK;
 }
+
+int hy_model_profile(hy_model h, int enable) { return hy_model_profile_classes(h, enable ? ~0u : 0u); }
 
 int hy_model_profile_read(hy_model h, int maxcls, int* launches, double* total_ms, double* bytes) {
   if (!h || !launches || !total_ms || !bytes) return set_error(HY_ERR_INVALID, "null argument");

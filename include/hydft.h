@@ -218,8 +218,12 @@ int hy_model_kernel(hy_model m, double k, double* h_K);
    every kernel launch of the model, on the model's stream; _read returns, per
    kernel class, the launch count, the summed event time (ms) and the summed
    algorithmic bytes (compulsory reads + writes of each launch), and returns
-   the number of classes.  Class names come from hy_profile_class_name. */
+   the number of classes.  Class names come from hy_profile_class_name.
+   hy_model_profile_classes(m, mask) records only the classes c whose bit
+   (1 << c) is set in mask (0: off): each event record costs a few us of
+   stream time, so a timed run brackets only the class it measures. */
 int hy_model_profile(hy_model m, int enable);
+int hy_model_profile_classes(hy_model m, unsigned mask);
 int hy_model_profile_read(hy_model m, int maxcls, int* launches, double* total_ms, double* bytes);
 const char* hy_profile_class_name(int cls);
 

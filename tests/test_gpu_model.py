@@ -189,3 +189,33 @@ def test_mixed_electrons_density_table_and_fallback(gpu):
     assert P.rel_l2(ng - P.N0, om.n - P.N0) <= MIXED_TOL_DN
     for d in range(3):
         assert P.rel_l2(momg[d], om.mom[d]) <= MIXED_TOL_DN, d
+
+
+def test_profile_classes(gpu):
+    """hy_model_profile records an event pair around every launch of a step;
+    hy_model_profile_classes(mask) only around the classes in mask, with the
+    same launch count and bytes for those classes."""
+    import ctypes as C
+    from hydft_amd import abi
+    p, gm, _, _ = build("ion3d_newt_visc")
+    lib = abi.lib().lib
+    names = [lib.hy_profile_class_name(c).decode() for c in range(16)]
+    xa = names.index("x_phaseA")
+
+    def table():
+        la, ms, by = (C.c_int * 16)(), (C.c_double * 16)(), (C.c_double * 16)()
+        ncls = lib.hy_model_profile_read(gm.h, 16, la, ms, by)
+        return {names[c]: (la[c], ms[c], by[c]) for c in range(ncls) if la[c]}
+
+    abi.call("hy_model_profile", gm.h, 1)
+    gm.advance(2)
+    full = table()
+    assert full["x_phaseA"][0] == 2 * 3 and len(full) > 4 and all(v[1] > 0 for v in full.values())
+    abi.call("hy_model_profile_classes", gm.h, 1 << xa)
+    gm.advance(2)
+    only = table()
+    assert list(only) == ["x_phaseA"] and only["x_phaseA"][0] == full["x_phaseA"][0]
+    assert only["x_phaseA"][2] == full["x_phaseA"][2] and only["x_phaseA"][1] > 0
+    abi.call("hy_model_profile", gm.h, 0)
+    gm.advance(1)
+    assert table() == {}
