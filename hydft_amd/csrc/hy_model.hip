@@ -390,36 +390,54 @@ __device__ inline void xfft(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, int lane, 
 // DC and Nyquist imaginary parts dropped); the lane's points of a(x) and
 // b(x) are returned in qa[q], qb[q] (qb may be null).  Lanes of a line
 // beyond the last (lok false) read nothing.  MA, MB: the inputs' compile-time
-// multipliers (see xin_gather), or -1.
+// multipliers (see xin_gather), or -1.  The transform is split in two:
+// x_inv_load gathers the first stage's operands into registers, x_inv_run
+// transforms them, so a caller can issue the next pair's loads before it
+// transforms the current pair and overlap their latency with that work.
+//
+// The first stage's operands: element e = j + r NB0 of the line for
+// j = lane + TPC b, held as i = b R0 + r.
+template <class T, int NX, int TPC>
+struct XInvIn {
+  static constexpr int R0 = prad<NX, false>(0), NB0 = NX / R0, NI = ((NB0 + TPC - 1) / TPC) * R0;
+  cpx<T> va[NI], vb[NI];
+  __device__ static int kk(int lane, int i) {
+    const int j = lane + TPC * (i / R0), e = j + (i % R0) * NB0;
+    return 2 * e > NX ? NX - e : e;
+  }
+};
 template <class T, int NX, int TPC = 64, int MA = -1, int MB = -1>
-__device__ inline void x_inv_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, const XIn* A, const XIn* B,
-                                  size_t nlines, size_t line, bool lok, int Kx, double dkx, int lane, T* qa, T* qb) {
+__device__ inline void x_inv_load(const XIn* A, const XIn* B, size_t nlines, size_t line, bool lok, int Kx,
+                                  double dkx, int lane, XInvIn<T, NX, TPC>& P) {
+  using P_t = XInvIn<T, NX, TPC>;
+  constexpr int NI = P_t::NI;
+  int kk[NI];
+  bool ok[NI];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int j = lane + TPC * (i / P_t::R0);
+    kk[i] = P_t::kk(lane, i);
+    ok[i] = lok && (P_t::NB0 % TPC == 0 || j < P_t::NB0) && kk[i] < Kx;
+  }
+  xin_gather<T, NI, MA>(A, nlines, line, Kx, kk, ok, P.va);
+  xin_gather<T, NI, MB>(B, nlines, line, Kx, kk, ok, P.vb);
+  xin_finish<T, NI, MA>(A, line, dkx, kk, ok, P.va);
+  xin_finish<T, NI, MB>(B, line, dkx, kk, ok, P.vb);
+}
+template <class T, int NX, int TPC = 64>
+__device__ inline void x_inv_run(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, const XInvIn<T, NX, TPC>& P, int lane,
+                                 T* qa, T* qb) {
+  using P_t = XInvIn<T, NX, TPC>;
 #pragma unroll
   for (int q = 0; q < XQ<NX, TPC>::Q; ++q) {  // slots of idle lanes stay defined
     qa[q] = 0;
     if (qb) qb[q] = 0;
   }
-  // the first stage's operands: element e = j + r NB0 of the line for
-  // j = lane + TPC b, held as i = b R0 + r
-  constexpr int R0 = prad<NX, false>(0), NB0 = NX / R0, NI = ((NB0 + TPC - 1) / TPC) * R0;
-  int kk[NI];
-  bool ok[NI];
-#pragma unroll
-  for (int i = 0; i < NI; ++i) {
-    const int j = lane + TPC * (i / R0), e = j + (i % R0) * NB0;
-    kk[i] = 2 * e > NX ? NX - e : e;
-    ok[i] = lok && (NB0 % TPC == 0 || j < NB0) && kk[i] < Kx;
-  }
-  cpx<T> va[NI], vb[NI];
-  xin_gather<T, NI, MA>(A, nlines, line, Kx, kk, ok, va);
-  xin_gather<T, NI, MB>(B, nlines, line, Kx, kk, ok, vb);
-  xin_finish<T, NI, MA>(A, line, dkx, kk, ok, va);
-  xin_finish<T, NI, MB>(B, line, dkx, kk, ok, vb);
   __syncthreads();  // the previous transform's readers of b0/b1 are done
   auto in = [&](int bb, int r, int e) -> cpx<T> {
-    const int i = bb * R0 + r;
-    cpx<T> a = va[i], b = vb[i];
-    if (kk[i] == 0 || 2 * kk[i] == NX) { a.y = 0; b.y = 0; }
+    const int i = bb * P_t::R0 + r, k = P_t::kk(lane, i);
+    cpx<T> a = P.va[i], b = P.vb[i];
+    if (k == 0 || 2 * k == NX) { a.y = 0; b.y = 0; }
     return 2 * e > NX ? mk<T>(a.x + b.y, -a.y + b.x)   // conj(a) + i conj(b)
                       : mk<T>(a.x - b.y, a.y + b.x);   // a + i b
   };
@@ -429,6 +447,13 @@ __device__ inline void x_inv_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, cons
     if (qb) qb[q] = v.y;
   };
   xfft<T, NX, true, false, TPC>(b0, b1, tw, lane, in, out);
+}
+template <class T, int NX, int TPC = 64, int MA = -1, int MB = -1>
+__device__ inline void x_inv_pair(cpx<T>* b0, cpx<T>* b1, const cpx<T>* tw, const XIn* A, const XIn* B,
+                                  size_t nlines, size_t line, bool lok, int Kx, double dkx, int lane, T* qa, T* qb) {
+  XInvIn<T, NX, TPC> P;
+  x_inv_load<T, NX, TPC, MA, MB>(A, B, nlines, line, lok, Kx, dkx, lane, P);
+  x_inv_run<T, NX, TPC>(b0, b1, tw, P, lane, qa, qb);
 }
 
 // Forward x transform of two real lines packed as va + i vb, where
@@ -483,6 +508,15 @@ struct XWave {
   __device__ static void inv_pair(Smem& s, const XIn* A, const XIn* B, size_t nlines, size_t line, bool lok,
                                   int Kx, double dkx, int g, int lw, T* qa, T* qb) {
     x_inv_pair<T, NX, TPC, MA, MB>(s.b0 + lw * NX, s.b1 + lw * NX, s.tw, A, B, nlines, line, lok, Kx, dkx, g, qa, qb);
+  }
+  using InvIn = XInvIn<T, NX, TPC>;
+  template <int MA = -1, int MB = -1>
+  __device__ static void inv_load(const XIn* A, const XIn* B, size_t nlines, size_t line, bool lok, int Kx,
+                                  double dkx, int g, InvIn& P) {
+    x_inv_load<T, NX, TPC, MA, MB>(A, B, nlines, line, lok, Kx, dkx, g, P);
+  }
+  __device__ static void inv_run(Smem& s, const InvIn& P, int g, int lw, T* qa, T* qb) {
+    x_inv_run<T, NX, TPC>(s.b0 + lw * NX, s.b1 + lw * NX, s.tw, P, g, qa, qb);
   }
   template <class F>
   __device__ static void fwd_pair(Smem& s, F&& val, void* oA, void* oB, size_t line, bool lok, int Kx, int g, int lw) {
@@ -597,22 +631,29 @@ __device__ __attribute__((always_inline)) inline void phaseA_body(const PhysArgs
   // functional below is evaluated in FP64
   // Transform pairs ordered so that few fields are live at once: the
   // functional needs n - n0, |grad n|^2 and lap n; the outputs need mom.
+  // The gradient path transforms three pairs before the functional; in the
+  // 3-D forms each of the second and third pairs' loads is issued before the
+  // previous pair is transformed, so their latency overlaps that work (the
+  // run-time form issues the second pair's after, within the 96 VGPRs of 5
+  // waves per SIMD in FP32; issuing the (m1, m2) loads before the functional
+  // as well spills).
   T dn[PTS], m0[PTS], g2[PTS], lap[PTS];
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { m0[p] = 0; g2[p] = 0; lap[p] = 0; }
   if (grad) {
-    X::template inv_pair<MN, MN>(sm, &a.n, &a.lap, nlines, line, lok, a.Kx, a.dkx, g, lw, dn, lap);
+    typename X::InvIn P0, P1;
+    X::template inv_load<MN, MN>(&a.n, &a.lap, nlines, line, lok, a.Kx, a.dkx, g, P0);
+    if constexpr (C::D3)
+      X::template inv_load<MK, MN>(&a.gx, nd >= 2 ? &a.gy : nullptr, nlines, line, lok, a.Kx, a.dkx, g, P1);
+    X::inv_run(sm, P0, g, lw, dn, lap);
+    if constexpr (!C::D3)
+      X::template inv_load<MK, MN>(&a.gx, nd >= 2 ? &a.gy : nullptr, nlines, line, lok, a.Kx, a.dkx, g, P1);
+    X::template inv_load<MN, MN>(nd >= 3 ? &a.gz : nullptr, &a.m[0], nlines, line, lok, a.Kx, a.dkx, g, P0);
     T gx[PTS], gy[PTS];
-#pragma unroll
-    for (int p = 0; p < PTS; ++p) gy[p] = 0;
-    X::template inv_pair<MK, MN>(sm, &a.gx, nd >= 2 ? &a.gy : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, gx,
-                                 gy);
+    X::inv_run(sm, P1, g, lw, gx, gy);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) g2[p] = gx[p] * gx[p] + gy[p] * gy[p];
-#pragma unroll
-    for (int p = 0; p < PTS; ++p) gx[p] = 0;
-    X::template inv_pair<MN, MN>(sm, nd >= 3 ? &a.gz : nullptr, &a.m[0], nlines, line, lok, a.Kx, a.dkx, g, lw,
-                                 gx, m0);
+    X::inv_run(sm, P0, g, lw, gx, m0);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) g2[p] += gx[p] * gx[p];
   } else {
@@ -733,10 +774,14 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
   T dn[PTS], gmx[PTS], gmy[PTS], gmz[PTS];
 #pragma unroll
   for (int p = 0; p < PTS; ++p) { gmy[p] = 0; gmz[p] = 0; }
-  X::template inv_pair<MN, MK>(sm, &a.n, &a.mux, nlines, line, lok, a.Kx, a.dkx, g, lw, dn, gmx);
-  if (nd >= 2)
-    X::template inv_pair<MN, MN>(sm, &a.muy, nd >= 3 ? &a.muz : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, gmy,
-                                 gmz);
+  {  // the second pair's loads are issued before the first pair is transformed
+    typename X::InvIn P0, P1;
+    X::template inv_load<MN, MK>(&a.n, &a.mux, nlines, line, lok, a.Kx, a.dkx, g, P0);
+    if (nd >= 2)
+      X::template inv_load<MN, MN>(&a.muy, nd >= 3 ? &a.muz : nullptr, nlines, line, lok, a.Kx, a.dkx, g, P1);
+    X::inv_run(sm, P0, g, lw, dn, gmx);
+    if (nd >= 2) X::inv_run(sm, P1, g, lw, gmy, gmz);
+  }
   // external force (external_potential.f90:65-93) at the stage-start time,
   // evaluated point by point.  The wave force amp k sin(kx x + ph), with the
   // line's phase ph = ky y + kz z - w t, is formed as
