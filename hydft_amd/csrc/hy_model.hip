@@ -830,16 +830,21 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
   if constexpr (MX) {
   if (!a.evolve) return;
   // Maxwell stress advection dPi_c = -sum_d u_d d_d Pi_c (model.f90:126-134),
-  // two components per pass
+  // two components per pass.  PF: each pair's loads are issued before the
+  // previous pair is transformed (FP32; the FP64 form, at 2 waves per SIMD
+  // either way, runs slower so)
+  constexpr bool PF = sizeof(T) == 4;
   T u0[PTS], u1[PTS], u2[PTS];
   {
     T m0[PTS], m1[PTS], m2[PTS];
 #pragma unroll
     for (int p = 0; p < PTS; ++p) { m1[p] = 0; m2[p] = 0; }
-    X::template inv_pair<MN, MN>(sm, &a.m[0], nd >= 2 ? &a.m[1] : nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw,
-                                 m0, m1);
-    if (nd >= 3)
-      X::template inv_pair<MN, MN>(sm, &a.m[2], nullptr, nlines, line, lok, a.Kx, a.dkx, g, lw, m2, nullptr);
+    typename X::InvIn Q0, Q1;
+    X::template inv_load<MN, MN>(&a.m[0], nd >= 2 ? &a.m[1] : nullptr, nlines, line, lok, a.Kx, a.dkx, g, Q0);
+    if (PF && nd >= 3) X::template inv_load<MN, MN>(&a.m[2], nullptr, nlines, line, lok, a.Kx, a.dkx, g, Q1);
+    X::inv_run(sm, Q0, g, lw, m0, m1);
+    if (!PF && nd >= 3) X::template inv_load<MN, MN>(&a.m[2], nullptr, nlines, line, lok, a.Kx, a.dkx, g, Q1);
+    if (nd >= 3) X::inv_run(sm, Q1, g, lw, m2, nullptr);
 #pragma unroll
     for (int p = 0; p < PTS; ++p) {
       const double inv = fd::rcp_nr(a.mass * (a.n0 + dn[p]));
@@ -853,10 +858,17 @@ __device__ __attribute__((always_inline)) inline void phaseB_body(const PhysArgs
     if (nd == 1) {
       X::inv_pair(sm, &a.px[c], &a.px[c + 1], nlines, line, lok, a.Kx, a.dkx, g, lw, xa, xb);
     } else {
-      X::template inv_pair<MK, MN>(sm, &a.px[c], &a.py[c], nlines, line, lok, a.Kx, a.dkx, g, lw, xa, ya);
-      X::template inv_pair<MK, MN>(sm, &a.px[c + 1], &a.py[c + 1], nlines, line, lok, a.Kx, a.dkx, g, lw, xb, yb);
-      if (nd >= 3)
-        X::template inv_pair<MN, MN>(sm, &a.pz[c], &a.pz[c + 1], nlines, line, lok, a.Kx, a.dkx, g, lw, za, zb);
+      typename X::InvIn Q0, Q1;
+      X::template inv_load<MK, MN>(&a.px[c], &a.py[c], nlines, line, lok, a.Kx, a.dkx, g, Q0);
+      if (PF) X::template inv_load<MK, MN>(&a.px[c + 1], &a.py[c + 1], nlines, line, lok, a.Kx, a.dkx, g, Q1);
+      X::inv_run(sm, Q0, g, lw, xa, ya);
+      if (!PF) X::template inv_load<MK, MN>(&a.px[c + 1], &a.py[c + 1], nlines, line, lok, a.Kx, a.dkx, g, Q1);
+      if (PF && nd >= 3)
+        X::template inv_load<MN, MN>(&a.pz[c], &a.pz[c + 1], nlines, line, lok, a.Kx, a.dkx, g, Q0);
+      X::inv_run(sm, Q1, g, lw, xb, yb);
+      if (!PF && nd >= 3)
+        X::template inv_load<MN, MN>(&a.pz[c], &a.pz[c + 1], nlines, line, lok, a.Kx, a.dkx, g, Q0);
+      if (nd >= 3) X::inv_run(sm, Q0, g, lw, za, zb);
     }
     X::fwd_pair(
         sm,
