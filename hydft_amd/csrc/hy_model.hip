@@ -2607,6 +2607,12 @@ int hy_model_set_state(hy_model h, const double* h_n, const double* h_mom3, cons
       if (h_pi6) rc = put(h_pi6 + nr * c, 4 + c);
       else rc = hipMemsetAsync(sfield(m, 4 + c), 0, sizeof(c64) * m.nt, m.st) == hipSuccess ? HY_OK : HY_ERR_HIP;
     }
+  // a new state starts clean: the non-positive-density and no-time-step
+  // conditions seen on the previous state are cleared
+  const int clear2[2] = {0, 0};
+  static_assert(offsetof(Ctl, nodt) == offsetof(Ctl, nonpos) + sizeof(int), "Ctl: nonpos, nodt adjacent");
+  if (rc == HY_OK && hipMemcpyAsync(&m.ctl->nonpos, clear2, sizeof(clear2), hipMemcpyHostToDevice, m.st) != hipSuccess)
+    rc = set_error(HY_ERR_HIP, "hy_model_set_state: control block");
   (void)hipStreamSynchronize(m.st);
   m.relax_pending = 0;
   return rc;
@@ -2668,7 +2674,9 @@ int hy_model_get_time(hy_model h, double* t, double* dt, int* step) {
 int hy_model_set_dt(hy_model h, double dt) {
   if (!h) return set_error(HY_ERR_INVALID, "null argument");
   Model& m = h->m;
+  const int clear = 0;  // a given step clears an earlier no-time-step condition
   HY_HIP(hipMemcpyAsync(&m.ctl->dt, &dt, sizeof(double), hipMemcpyHostToDevice, m.st));
+  HY_HIP(hipMemcpyAsync(&m.ctl->nodt, &clear, sizeof(int), hipMemcpyHostToDevice, m.st));
   HY_HIP(hipStreamSynchronize(m.st));
   m.h_dt = dt;
   return HY_OK;

@@ -184,7 +184,9 @@ typedef struct {
 
 int hy_model_create(const hy_model_desc* desc, hy_model* out);
 int hy_model_destroy(hy_model m);
-/* state in the reference layout; projected onto the kept band on entry */
+/* state in the reference layout; projected onto the kept band on entry.
+   A new state clears the non-positive-density and no-time-step conditions
+   of the previous one; hy_model_set_dt clears the no-time-step condition. */
 int hy_model_set_state(hy_model m, const double* h_n, const double* h_mom3, const double* h_pi6);
 int hy_model_get_state(hy_model m, double* h_n, double* h_mom3, double* h_pi6);
 int hy_model_set_time(hy_model m, double t, int step);

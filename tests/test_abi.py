@@ -60,3 +60,34 @@ end program
     want = [C.sizeof(abi.TermDesc), C.sizeof(abi.ModelDesc), abi.ModelDesc.terms.offset,
             abi.ModelDesc.vext_mode.offset, abi.ModelDesc.cfl.offset, abi.TermDesc.c_tab.offset]
     assert got == want
+
+
+def test_grid_validation_matches_reference():
+    """hy_grid_create rejects the grids grid_init stops on (grid.f90:45-48), with
+    its messages, before any device call; lengths outside the supported radix
+    set are HY_ERR_UNSUPPORTED."""
+    h = C.c_void_p()
+
+    def err(nx, ny, nz):
+        try:
+            abi.call("hy_grid_create", nx, ny, nz, 1.0, 1.0, 1.0, 1, abi.HY_PREC_FP64, C.byref(h))
+        except abi.HydftError as e:
+            return e.code, str(e)
+        raise AssertionError(f"grid {nx}x{ny}x{nz} accepted")
+
+    code, msg = err(1, 1, 1)
+    assert code == abi.HY_ERR_INVALID and "nx must be >= 2" in msg
+    assert err(0, 8, 8)[0] == abi.HY_ERR_INVALID
+    code, msg = err(7, 1, 1)
+    assert code == abi.HY_ERR_INVALID and "nx must be even" in msg
+    code, msg = err(8, 3, 1)
+    assert code == abi.HY_ERR_INVALID and "ny must be even (or 1)" in msg
+    code, msg = err(8, 8, 5)
+    assert code == abi.HY_ERR_INVALID and "nz must be even (or 1)" in msg
+    for shape in [(6, 1, 1), (10, 10, 10), (512, 1, 1), (16, 20, 1), (16, 16, 40)]:
+        assert err(*shape)[0] == abi.HY_ERR_UNSUPPORTED, shape
+    try:
+        abi.call("hy_grid_create", 16, 1, 1, 1.0, 1.0, 1.0, 1, 7, C.byref(h))
+        raise AssertionError("unknown precision accepted")
+    except abi.HydftError as e:
+        assert e.code == abi.HY_ERR_INVALID
