@@ -17,6 +17,7 @@
 // Real fields never leave LDS/registers: each x kernel inverse-transforms its
 // inputs along x, does the pointwise physics and forward-transforms the results.
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <mutex>
 #include <vector>
