@@ -47,9 +47,10 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import abi as _abi
 from .host import hnc as _hnc
 from .host.analysis import fit_damped_mode, fit_driven_response
-from .host.initial import set_initial_condition
+from .host.initial import InitialError, set_initial_condition
 from .host.params import Params
 
 TWOPI = 2.0 * math.pi
@@ -77,6 +78,7 @@ class Result:
     steps: int
     probe: np.ndarray = field(repr=False)
     value: dict = field(default_factory=dict)
+    error: str = ""              # the run's error (a non-positive density, no time step), if any
 
     @property
     def latency(self) -> float:
@@ -211,12 +213,24 @@ class Server:
         # the budget needs their memory): freeing device memory waits for all
         # queued work on the device and would stall the other runs' launches
         self._retired = []
+        self._failed = []  # (request, admission time, message, code) of setups that failed
 
     def _admit(self, reqs, now: float) -> list:
         """Start `reqs` (one batch key) as one batch, or each as a captured run."""
+        # a request whose setup fails (its initial density not positive, say)
+        # fails alone: it is reported by serve() and the others start
+        models, ok = [], []
+        for r in reqs:
+            try:
+                models.append(make_model(r, self.precision, self.cache))
+                ok.append(r)
+            except (InitialError, _abi.HydftError) as e:
+                self._failed.append((r, now, str(e), getattr(e, "code", _abi.HY_ERR_INVALID)))
+        reqs = ok
+        if not reqs:
+            return []
         req = reqs[0]
         dt_every = DT_EVERY if req.p.dt <= 0.0 else 0
-        models = [make_model(r, self.precision, self.cache) for r in reqs]
         # each run is charged the larger of the estimate it was admitted on and its actual bytes
         charge = [max(m.device_bytes(), estimate_bytes(r.p, self.precision)) for m, r in zip(models, reqs)]
         # same-key runs can still differ in their kept box (a 2/3-rule boundary
@@ -300,6 +314,10 @@ class Server:
                 for s in self._admit(group, now):
                     active.append(s)
                     used += s.bytes
+                for r, at, msg, code in self._failed:
+                    results.append(Result(r.rid, r.kind, r.arrival, at, time.perf_counter() - t0, 0,
+                                          np.zeros((0, 6)), {"code": code}, msg))
+                self._failed = []
                 self.peak_active = max(self.peak_active, sum(len(s.models) for s in active))
                 self.peak_bytes = max(self.peak_bytes, used)
                 progressed = True
@@ -307,7 +325,12 @@ class Server:
                 if not s.idle():
                     continue
                 for i, (req, m) in enumerate(zip(s.reqs, s.models)):
-                    m.check()
+                    try:
+                        m.check()
+                    except _abi.HydftError as e:  # this request failed; the others go on
+                        results.append(Result(req.rid, req.kind, req.arrival, s.start, time.perf_counter() - t0,
+                                              0, np.zeros((0, 6)), {"code": e.code}, str(e)))
+                        continue
                     rows = s.rows(i)[: req.nsteps]
                     done = time.perf_counter() - t0
                     results.append(Result(req.rid, req.kind, req.arrival, s.start, done, len(rows), rows,
@@ -343,8 +366,16 @@ def one_at_a_time(requests, precision: str = "mixed", cache=None):
     t0 = time.perf_counter()
     for req in sorted(requests, key=lambda r: r.arrival):
         start = time.perf_counter() - t0
-        m = make_model(req, precision, cache)
-        taken, rows = m.advance(req.nsteps, dt_every=DT_EVERY if req.p.dt <= 0.0 else 0, probe=req.mode)
+        m = None
+        try:
+            m = make_model(req, precision, cache)
+            taken, rows = m.advance(req.nsteps, dt_every=DT_EVERY if req.p.dt <= 0.0 else 0, probe=req.mode)
+        except (InitialError, _abi.HydftError) as e:  # this request failed; the others go on
+            results.append(Result(req.rid, req.kind, req.arrival, start, time.perf_counter() - t0, 0,
+                                  np.zeros((0, 6)), {"code": getattr(e, "code", _abi.HY_ERR_INVALID)}, str(e)))
+            if m is not None:
+                m.close()
+            continue
         done = time.perf_counter() - t0
         results.append(Result(req.rid, req.kind, req.arrival, start, done, taken, rows,
                               analyse(req, rows, linear_for_model(m))))
@@ -355,7 +386,8 @@ def one_at_a_time(requests, precision: str = "mixed", cache=None):
 def summary(results, wall: float) -> dict:
     lat = np.array([r.latency for r in results])
     out = {"runs": len(results), "wall_s": wall, "runs_per_s": len(results) / wall,
-           "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99))}
+           "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+           "failed": sum(1 for r in results if r.error)}
     dev = [r.value["chi_rel_dev"] for r in results if "chi_rel_dev" in r.value]
     if dev:
         out["chi_vs_linear_median"] = float(np.median(dev))

@@ -157,3 +157,32 @@ def test_batch_holds_the_probe_ring(gpu):
     assert np.all(np.isfinite(ms[0].graph_probe()[:20]))
     for m in ms:
         m.close()
+
+
+@pytest.mark.gpu
+def test_a_failed_request_does_not_stop_the_others(gpu):
+    """Requests that fail are reported alone: one whose density turns
+    non-positive in its steps (a mode of 0.9 n0 stepped with a fixed dt far
+    above the stability limit) with the reference's message (model.f90:91),
+    and one whose initial density is not positive (refused at setup); the
+    request served beside them gives the result it gives alone."""
+    good = _req(0, N=16)
+    bad = _req(1, N=16)
+    bad.kind, bad.mode = "dispersion", (1, 0, 0)
+    bad.p.vext, bad.p.vext_amplitude = "none", 0.0
+    bad.p.ic_amplitude, bad.p.dt = 0.9, 50.0
+    worse = _req(2, N=16)
+    worse.kind, worse.mode = "dispersion", (1, 0, 0)
+    worse.p.vext, worse.p.vext_amplitude = "none", 0.0
+    worse.p.ic_amplitude = 1.5
+    cache = serve._hnc.StructureCache()
+    alone = serve.Server(precision="fp64", cache=cache).serve([good], realtime=False)[0]
+    srv = serve.Server(precision="fp64", cache=cache, max_slots=4)
+    out = {r.rid: r for r in srv.serve([good, bad, worse], realtime=False)}
+    assert out[1].error and "non-positive" in out[1].error and out[1].steps == 0
+    assert out[2].error and "not positive" in out[2].error
+    assert not out[0].error and np.array_equal(out[0].probe, alone.probe)
+    assert serve.summary(list(out.values()), 1.0)["failed"] == 2
+    one = {r.rid: r for r in serve.one_at_a_time([good, bad, worse], precision="fp64", cache=cache)}
+    assert one[1].error and one[2].error and not one[0].error
+    assert np.array_equal(one[0].probe, alone.probe)
