@@ -495,12 +495,30 @@ __host__ __device__ constexpr int x_tpc(int NX) {
   while (t < 64 && ((NX / radix_of(NX, nstages_of(NX) - 1)) + t - 1) / t * radix_of(NX, nstages_of(NX) - 1) > 4) t *= 2;
   return t;
 }
+// every Stockham stage of an NX-point line on TPC lanes has at most one
+// butterfly per lane
+template <int NX, int TPC>
+__host__ __device__ constexpr bool x_one_butterfly_per_lane() {
+  for (int s = 0; s < nstages_of(NX); ++s)
+    if (NX / radix_of(NX, s) > TPC) return false;
+  return true;
+}
 template <class T, int NX>
 struct XWave {
   static constexpr int TPC = x_tpc(NX), LPW = 64 / TPC, PTS = XQ<NX, TPC>::Q;
+  // One LDS buffer serves both sides of the Stockham exchange when each
+  // stage has one butterfly per lane: the workgroup is one wave, a stage's
+  // writes depend on all of its reads (they wait for the read data), and the
+  // LDS executes a wave's accesses in order, so a stage may overwrite what it
+  // read.  With two butterflies per lane the first one's writes would precede
+  // the second one's reads, and the stages ping-pong between two buffers.
+  // (A 256-point line: 4 KiB of LDS per wave instead of 6, which lets the
+  // 66-VGPR phase B hold 7 waves per SIMD.)
+  static constexpr bool ONEBUF = x_one_butterfly_per_lane<NX, TPC>();
   struct Smem {
-    cpx<T> b0[LPW * NX], b1[LPW * NX], tw[NX];
+    cpx<T> b0[LPW * NX], b1[ONEBUF ? 1 : LPW * NX], tw[NX];
   };
+  __device__ static cpx<T>* buf1(Smem& s, int lw) { return ONEBUF ? s.b0 + lw * NX : s.b1 + lw * NX; }
   __device__ static void init(Smem& s, const cpx<T>* twg, int lane) {
     for (int j = lane; j < NX; j += 64) s.tw[swz(j)] = twg[j];
   }
@@ -508,7 +526,7 @@ struct XWave {
   template <int MA = -1, int MB = -1>
   __device__ static void inv_pair(Smem& s, const XIn* A, const XIn* B, size_t nlines, size_t line, bool lok,
                                   int Kx, double dkx, int g, int lw, T* qa, T* qb) {
-    x_inv_pair<T, NX, TPC, MA, MB>(s.b0 + lw * NX, s.b1 + lw * NX, s.tw, A, B, nlines, line, lok, Kx, dkx, g, qa, qb);
+    x_inv_pair<T, NX, TPC, MA, MB>(s.b0 + lw * NX, buf1(s, lw), s.tw, A, B, nlines, line, lok, Kx, dkx, g, qa, qb);
   }
   using InvIn = XInvIn<T, NX, TPC>;
   template <int MA = -1, int MB = -1>
@@ -517,11 +535,11 @@ struct XWave {
     x_inv_load<T, NX, TPC, MA, MB>(A, B, nlines, line, lok, Kx, dkx, g, P);
   }
   __device__ static void inv_run(Smem& s, const InvIn& P, int g, int lw, T* qa, T* qb) {
-    x_inv_run<T, NX, TPC>(s.b0 + lw * NX, s.b1 + lw * NX, s.tw, P, g, qa, qb);
+    x_inv_run<T, NX, TPC>(s.b0 + lw * NX, buf1(s, lw), s.tw, P, g, qa, qb);
   }
   template <class F>
   __device__ static void fwd_pair(Smem& s, F&& val, void* oA, void* oB, size_t line, bool lok, int Kx, int g, int lw) {
-    x_fwd_pair<T, NX, TPC>(s.b0 + lw * NX, s.b1 + lw * NX, s.tw, val, oA, oB, line, lok, Kx, g);
+    x_fwd_pair<T, NX, TPC>(s.b0 + lw * NX, buf1(s, lw), s.tw, val, oA, oB, line, lok, Kx, g);
   }
   // sums / maxima over the TPC lanes of a line (aligned groups of the wave)
   __device__ static double line_sum(double v) {
